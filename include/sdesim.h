@@ -1,0 +1,136 @@
+/* sdesim.h -- C ABI of the MI355X SDE path simulator (libsdesim.so).
+ *
+ * The reference exposes two surfaces: the PyO3 function sde_sim_rs.simulate
+ * (src/py_binding.rs:8-56) and the Rust crate API sim::simulate (src/sim/mod.rs:20-92) with
+ * proc::util::parse_equations (src/proc/util.rs:52-66). Per-step trait calls cannot cross to a
+ * GPU, so this ABI is coarse: parse once, plan once (compile + upload constants), run many times.
+ *
+ * Conventions: plain C types and pointers only; every function that can fail returns an int
+ * status (SDESIM_OK, SDESIM_EVALUE for what the reference reports as ValueError / a parse error,
+ * SDESIM_ERUNTIME for a RuntimeError or a device failure, SDESIM_EPANIC where the reference
+ * panics: unknown scheme, undefined variable, RK without a stochastic increment, Sobol
+ * dimension or length beyond the table, empty time grid) and leaves a message in a
+ * thread-local buffer read by sdesim_last_error(). No C++ exception crosses the ABI. Handles are
+ * owned by the library and freed by the caller with the matching free function. Distinct
+ * handles may be used from different threads concurrently; the run-time compiled kernel cache
+ * is guarded by a mutex. All device work of a call is issued on the given HIP stream (NULL =
+ * the null stream); functions that return host data synchronise that stream.
+ */
+#ifndef SDESIM_H
+#define SDESIM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDESIM_OK 0
+#define SDESIM_EVALUE 1
+#define SDESIM_ERUNTIME 2
+#define SDESIM_EPANIC 3 /* a condition under which the reference panics (PyO3 PanicException) */
+
+#define SDESIM_EULER 0
+#define SDESIM_RUNGE_KUTTA 1
+
+#define SDESIM_PSEUDO 0   /* ChaCha8Rng::seed_from_u64(seed + s) per path (src/rng/pseudo.rs) */
+#define SDESIM_SOBOL 1    /* Sobol points; scramble selects the randomisation (src/rng/sobol.rs) */
+#define SDESIM_INJECTED 2 /* caller-provided uniforms [N*m][S] (test hook, no reference counterpart) */
+
+#define SDESIM_SCRAMBLE_SHIFT 0 /* the reference's per-path f64 random shift (src/rng/sobol.rs:62-79) */
+#define SDESIM_SCRAMBLE_XOR 1   /* digital XOR shift shared by all paths of a replicate */
+
+#define SDESIM_F64 0
+#define SDESIM_F32 1
+
+#define SDESIM_COMPAT_REFERENCE 0 /* the reference's name-cache semantics (stale RK / algebraic values) */
+#define SDESIM_COMPAT_TEXTBOOK 1  /* coefficients see the settled rows */
+
+#define SDESIM_OUT_FILTRATION 1
+#define SDESIM_OUT_TERMINAL 2
+
+typedef struct sdesim_universe sdesim_universe;
+typedef struct sdesim_plan sdesim_plan;
+
+typedef struct sdesim_options {
+  int32_t scheme;         /* SDESIM_EULER | SDESIM_RUNGE_KUTTA */
+  int32_t rng;            /* SDESIM_PSEUDO | SDESIM_SOBOL | SDESIM_INJECTED */
+  int32_t scramble;       /* SDESIM_SCRAMBLE_SHIFT | SDESIM_SCRAMBLE_XOR (Sobol only) */
+  int32_t precision;      /* SDESIM_F64 | SDESIM_F32 (state, coefficients and filtration values) */
+  int32_t compat;         /* SDESIM_COMPAT_REFERENCE | SDESIM_COMPAT_TEXTBOOK */
+  int32_t outputs;        /* bit mask of SDESIM_OUT_* */
+  int32_t paths_per_lane; /* 0 = default */
+  int32_t block_size;     /* 0 = default (256) */
+  uint64_t seed;          /* path s uses seed + s (src/sim/mod.rs:56,65) */
+  uint64_t scramble_seed; /* XOR masks = words of ChaCha8Rng::seed_from_u64(scramble_seed) */
+  int64_t sobol_first;    /* point index of path 0; < 0 = default (5 for shift, 0 for XOR) */
+} sdesim_options;
+
+/* Fill `o` with the defaults: Euler, pseudo, f64, reference compat, filtration output. */
+void sdesim_default_options(sdesim_options* o);
+
+/* Replaces proc::util::parse_equations (src/proc/util.rs:52-66) and ProcessUniverse::new
+ * (src/proc/mod.rs:70-89). Parse errors return SDESIM_EVALUE with the reference's message. */
+int sdesim_parse_equations(const char* const* equations, int32_t n_equations, sdesim_universe** out);
+void sdesim_free_universe(sdesim_universe* u);
+
+/* ProcessUniverse accessors (src/proc/mod.rs:61-68). */
+int32_t sdesim_num_processes(const sdesim_universe* u);
+int32_t sdesim_num_increments(const sdesim_universe* u); /* stochastic_registry.len() */
+const char* sdesim_process_name(const sdesim_universe* u, int32_t p);
+int32_t sdesim_process_is_algebraic(const sdesim_universe* u, int32_t p);
+int32_t sdesim_process_index(const sdesim_universe* u, const char* name); /* registry lookup; -1 if absent */
+
+/* Plan a run of sim::simulate (src/sim/mod.rs:20-92) for a fixed time grid, initial row and
+ * path count: generates and compiles the fused kernel (hipRTC, cached by source text) and
+ * uploads the constant inputs. `x0` holds one initial value per process (the caller resolves
+ * names as ScenarioFiltration::new does, src/filtration.rs:42-50). `payoff` (nullable) is an
+ * expression over process names and t evaluated on the terminal row. */
+int sdesim_plan_create(const sdesim_universe* u, const double* times, int32_t n_times, const double* x0,
+                       int64_t scenarios, const sdesim_options* opt, const char* payoff, sdesim_plan** out);
+void sdesim_plan_free(sdesim_plan* plan);
+
+/* Bytes of the [T][P][S] filtration buffer the plan writes (4 or 8 bytes per value). */
+int64_t sdesim_plan_filtration_bytes(const sdesim_plan* plan);
+
+/* Run the plan on `stream`: the kernel writes the step-major filtration [T][P][S] into
+ * d_filtration (device, may be NULL when SDESIM_OUT_FILTRATION is off) and, with
+ * SDESIM_OUT_TERMINAL, reduces terminal moments into the plan. `d_injected` ([N*m][S] f64,
+ * device) is read only with SDESIM_INJECTED. Asynchronous. */
+int sdesim_plan_run(sdesim_plan* plan, void* d_filtration, const double* d_injected, void* stream);
+
+/* Terminal moments of the last run: for each process p (and the payoff last, if any)
+ * out[2q] = mean, out[2q+1] = sample variance (ddof 1). Synchronises the stream. */
+int sdesim_plan_moments(sdesim_plan* plan, double* h_out, void* stream);
+
+/* The reference's long layout (src/filtration.rs:81-116, rows ordered scenario -> time ->
+ * process, src/sim/mod.rs:88-91): transpose a [T][P][S] device filtration (f32 or f64) into
+ * the value column [S][T][P] as f64 on the device. */
+int sdesim_filtration_to_long(const void* d_filtration, int32_t precision, int64_t scenarios, int32_t n_times,
+                              int32_t n_processes, double* d_long, void* stream);
+
+/* One-call equivalent of the PyO3 simulate (src/py_binding.rs:8-56) minus the DataFrame
+ * assembly: validates, plans, runs and writes the long value column [S][T][P] (f64) to host
+ * memory. Initial values are given by name as in the reference. */
+int sdesim_simulate_long(const sdesim_universe* u, const double* times, int32_t n_times,
+                         const char* const* init_names, const double* init_values, int32_t n_init,
+                         int64_t scenarios, const sdesim_options* opt, double* h_values, void* stream);
+
+/* Generated HIP source of a plan's kernel (for inspection); valid while the plan lives. */
+const char* sdesim_plan_source(const sdesim_plan* plan);
+
+/* Host-only helpers (no device needed): the kernel source a plan would generate for a time grid
+ * of n_times points, and a gfx950 compile of a source through hipRTC. `needed` receives the
+ * buffer size including the terminating NUL. */
+int sdesim_generate_source(const sdesim_universe* u, int32_t n_times, const sdesim_options* opt, const char* payoff,
+                           char* buf, int64_t buf_len, int64_t* needed);
+int sdesim_compile_source(const char* source, int64_t* code_bytes);
+
+const char* sdesim_last_error(void);
+int32_t sdesim_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SDESIM_H */

@@ -1,0 +1,36 @@
+// codegen.hpp -- HIP source for one universe + options (compiled at run time with hipRTC).
+#pragma once
+#include <string>
+#include <vector>
+
+#include "universe.hpp"
+
+namespace sdesim {
+
+struct CodegenOptions {
+  int scheme = 0;       // 0 Euler-Maruyama, 1 "Runge-Kutta first order"
+  int compat = 0;       // 0 reference cache semantics, 1 textbook (refresh from settled rows)
+  bool f32 = false;     // f32 state and coefficients (f64 uniforms and estimator sums)
+  int rng = 0;          // SDE_RNG_* in sdesim_device.hpp
+  int out = 1;          // SDE_OUT_* bit mask
+  int ppl = 1;          // paths per lane
+  int block = 256;      // threads per block
+  bool steps = true;    // false when the time grid has a single point (nothing is evaluated)
+  std::string payoff;   // expression over process names and t at the last time point
+};
+
+struct UndefinedName : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// A condition under which the reference panics (surfaced by PyO3 as PanicException).
+struct Panic : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// Returns the complete kernel source (entry point "sdesim_kernel"). Throws ParseError for a bad
+// payoff, UndefinedName when an evaluated expression names an unknown variable, and
+// Panic for the reference's run-time panics (RK with no stochastic increment).
+std::string generate_kernel_source(const Universe& u, const CodegenOptions& o);
+
+}  // namespace sdesim

@@ -1,0 +1,41 @@
+// sdesim_args.hpp -- the argument block of the fused path kernel and its mode constants.
+// Plain C++ (no HIP): included by the host runtime and by the device code (ahead-of-time and
+// hipRTC builds), so both sides agree on the layout.
+#pragma once
+
+typedef unsigned int sde_u32;
+typedef unsigned long long sde_u64;
+
+// ------------------------------------------------------------------------------------------
+// Kernel argument block (identical layout on host and device; 8-byte fields first)
+// ------------------------------------------------------------------------------------------
+struct SimArgs {
+  const double* times;       // [T]
+  const double* dts;         // [N]  t[n+1] - t[n]
+  const double* sdts;        // [N]  sqrt(dt)
+  const float* times_f;      // [T]  f32 copies for f32 kernels
+  const float* dts_f;        // [N]
+  const float* sdts_f;       // [N]
+  const double* x0;          // [P]
+  const sde_u32* sobol_tab;  // [dims][64 + 256 * (nslices - 1)] sliced XOR tables
+  const sde_u32* xor_mask;   // [dims] digital-shift masks (XOR mode)
+  const double* injected;    // [N * M][S] injected uniforms (test hook)
+  void* filtration;          // [T][P][S] real, or null
+  double* partials;          // [gridDim.x][2 * KQ] (sum, sum of squares) of shifted values, or null
+  long long S;               // paths
+  sde_u64 seed;              // path s uses seed + s (wrapping)
+  long long sobol_first;     // Sobol point index of path 0
+  int N;                     // steps
+  int T;                     // time points
+  int nslices;               // Sobol table slices
+  int pad_;
+};
+
+#define SDE_RNG_PSEUDO 0
+#define SDE_RNG_SOBOL_SHIFT 1
+#define SDE_RNG_SOBOL_XOR 2
+#define SDE_RNG_INJECTED 3
+#define SDE_RNG_NONE 4
+
+#define SDE_OUT_FILTRATION 1
+#define SDE_OUT_TERMINAL 2

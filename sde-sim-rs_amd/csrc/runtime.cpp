@@ -1,0 +1,564 @@
+// runtime.cpp -- host runtime and C ABI (include/sdesim.h).
+//
+// Replaces sim::simulate's orchestration (src/sim/mod.rs:20-92): instead of a rayon loop over
+// scenarios that clones the universe, builds an RNG per scenario and steps it through boxed
+// trait objects, a plan generates the fused kernel for the universe (hipRTC, cached by source
+// text), uploads the time grid, initial row, Sobol tables and XOR masks once, and every run is
+// one kernel launch over all paths (plus a one-block reduction when terminal moments are asked).
+#include <hip/hip_runtime_api.h>
+#include <hip/hiprtc.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <set>
+#include <string>
+#include <vector>
+
+#include "../../include/sdesim.h"
+#include "codegen.hpp"
+#include "device/sdesim_args.hpp"
+#include "universe.hpp"
+
+// Embedded at build time: the device header text (for hipRTC) and the Joe-Kuo D6 table.
+extern "C" const char* sdesim_embedded_device_header;
+extern "C" const char* sdesim_embedded_args_header;
+extern "C" const int sdesim_jk_dims;
+extern "C" const unsigned int sdesim_jk_poly[];
+extern "C" const unsigned int sdesim_jk_vinit[];  // [dims][18]
+
+// Ahead-of-time kernels (aot_kernels.hip).
+hipError_t sdesim_launch_finalize(const double* partials, int nblocks, int kq, long long S, const double* shifts,
+                                  double* out, hipStream_t stream);
+hipError_t sdesim_launch_to_long(const void* filtration, int f32, long long S, int rows, double* out,
+                                 hipStream_t stream);
+
+struct sdesim_universe {
+  sdesim::Universe u;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Failure {
+  int code;
+  std::string msg;
+};
+
+[[noreturn]] void fail(int code, const std::string& m) { throw Failure{code, m}; }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) fail(SDESIM_ERUNTIME, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class F>
+int guarded(F&& f) {
+  try {
+    f();
+    return SDESIM_OK;
+  } catch (const Failure& e) {
+    g_last_error = e.msg;
+    return e.code;
+  } catch (const sdesim::ParseError& e) {
+    g_last_error = e.what();
+    return SDESIM_EVALUE;
+  } catch (const sdesim::UndefinedName& e) {
+    g_last_error = std::string("called `Result::unwrap()` on an `Err` value: ") + e.what();
+    return SDESIM_EPANIC;
+  } catch (const sdesim::Panic& e) {
+    g_last_error = e.what();
+    return SDESIM_EPANIC;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return SDESIM_ERUNTIME;
+  }
+}
+
+// ------------------------------------------------------------------------------------------
+// Kernel cache (hipRTC)
+// ------------------------------------------------------------------------------------------
+struct CompiledKernel {
+  hipModule_t module = nullptr;
+  hipFunction_t function = nullptr;
+};
+
+std::mutex g_cache_mu;
+std::map<std::pair<int, std::string>, CompiledKernel> g_kernels;
+
+// Compile generated source to a gfx950 code object (needs no device).
+std::vector<char> rtc_compile(const std::string& src) {
+  hiprtcProgram prog;
+  const char* hdrs[] = {sdesim_embedded_device_header, sdesim_embedded_args_header};
+  const char* names[] = {"sdesim_device.hpp", "sdesim_args.hpp"};
+  if (hiprtcCreateProgram(&prog, src.c_str(), "sdesim_kernel.hip", 2, hdrs, names) != HIPRTC_SUCCESS)
+    fail(SDESIM_ERUNTIME, "hiprtcCreateProgram failed");
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+  hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  if (r != HIPRTC_SUCCESS) {
+    size_t n = 0;
+    hiprtcGetProgramLogSize(prog, &n);
+    std::string log(n, '\0');
+    if (n) hiprtcGetProgramLog(prog, &log[0]);
+    hiprtcDestroyProgram(&prog);
+    fail(SDESIM_ERUNTIME, "kernel compilation failed: " + log);
+  }
+  size_t code_size = 0;
+  hiprtcGetCodeSize(prog, &code_size);
+  std::vector<char> code(code_size);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  return code;
+}
+
+CompiledKernel compile_kernel(const std::string& src) {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lock(g_cache_mu);
+  auto key = std::make_pair(dev, src);
+  auto it = g_kernels.find(key);
+  if (it != g_kernels.end()) return it->second;
+  std::vector<char> code = rtc_compile(src);
+  CompiledKernel k;
+  hip_check(hipModuleLoadData(&k.module, code.data()), "hipModuleLoadData");
+  hip_check(hipModuleGetFunction(&k.function, k.module, "sdesim_kernel"), "hipModuleGetFunction");
+  g_kernels[key] = k;
+  return k;
+}
+
+// ------------------------------------------------------------------------------------------
+// Sobol direction numbers and sliced XOR tables
+// ------------------------------------------------------------------------------------------
+// Direction numbers V[d][k] (k = 0..31 multiplies bit k of the Gray code) from the Joe-Kuo
+// table by the Bratley-Fox recurrence m_k = 2a_1 m_{k-1} ^ ... ^ 2^{s-1}a_{s-1} m_{k-s+1} ^
+// 2^s m_{k-s} ^ m_{k-s}; dimension 0 is the van der Corput sequence (all m_k = 1).
+void direction_numbers(int d, unsigned int V[32]) {
+  unsigned long long m[33];
+  if (d == 0) {
+    for (int k = 1; k <= 32; ++k) m[k] = 1;
+  } else {
+    const unsigned int poly = sdesim_jk_poly[d];
+    int s = 0;
+    while ((poly >> (s + 1)) != 0u) ++s;
+    for (int k = 1; k <= s && k <= 32; ++k) m[k] = sdesim_jk_vinit[(size_t)d * 18 + (k - 1)];
+    for (int k = s + 1; k <= 32; ++k) {
+      unsigned long long v = m[k - s] ^ (m[k - s] << s);
+      for (int i = 1; i < s; ++i)
+        if ((poly >> (s - i)) & 1u) v ^= m[k - i] << i;
+      m[k] = v;
+    }
+  }
+  for (int k = 1; k <= 32; ++k) V[k - 1] = (unsigned int)(m[k] << (32 - k));
+}
+
+struct SobolTables {
+  int dims = 0, nslices = 0;
+  unsigned int* dev = nullptr;
+};
+std::mutex g_sobol_mu;
+std::map<std::pair<int, int>, SobolTables> g_sobol;  // (device, slices) -> largest tables built so far
+
+// Tables covering `dims` dimensions and Gray codes below 2^(6 + 8 (nslices - 1)).
+SobolTables sobol_tables(int dims, int nslices) {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lock(g_sobol_mu);
+  SobolTables& t = g_sobol[std::make_pair(dev, nslices)];
+  if (t.dev && t.dims >= dims) return t;
+  int want = std::max(dims, t.dims);
+  const int stride = 64 + 256 * (nslices - 1);
+  std::vector<unsigned int> host((size_t)want * stride);
+  for (int d = 0; d < want; ++d) {
+    unsigned int V[32];
+    direction_numbers(d, V);
+    unsigned int* row = &host[(size_t)d * stride];
+    for (int j = 0; j < 64; ++j) {
+      unsigned int x = 0;
+      for (int b = 0; b < 6; ++b)
+        if ((j >> b) & 1) x ^= V[b];
+      row[j] = x;
+    }
+    for (int sl = 1; sl < nslices; ++sl) {
+      for (int j = 0; j < 256; ++j) {
+        unsigned int x = 0;
+        for (int b = 0; b < 8; ++b) {
+          int bit = 6 + 8 * (sl - 1) + b;
+          if (bit < 32 && ((j >> b) & 1)) x ^= V[bit];
+        }
+        row[64 + 256 * (sl - 1) + j] = x;
+      }
+    }
+  }
+  // The previous tables may still be read by queued kernels; they are kept until exit.
+  unsigned int* d_tab = nullptr;
+  hip_check(hipMalloc(&d_tab, host.size() * sizeof(unsigned int)), "hipMalloc(sobol tables)");
+  hip_check(hipMemcpy(d_tab, host.data(), host.size() * sizeof(unsigned int), hipMemcpyHostToDevice),
+            "hipMemcpy(sobol tables)");
+  t.dev = d_tab;
+  t.dims = want;
+  t.nslices = nslices;
+  return t;
+}
+
+// Words of ChaCha8Rng::seed_from_u64(seed): the XOR masks of the digital shift.
+void chacha8_words(unsigned long long seed, size_t n, std::vector<unsigned int>& out) {
+  unsigned int key[8];
+  unsigned long long st = seed;
+  for (int i = 0; i < 8; ++i) {
+    st = st * 6364136223846793005ULL + 11634580027462260723ULL;
+    unsigned int xs = (unsigned int)(((st >> 18) ^ st) >> 27);
+    unsigned int rot = (unsigned int)(st >> 59);
+    key[i] = (xs >> rot) | (xs << ((32u - rot) & 31u));
+  }
+  out.resize(n);
+  auto rotl = [](unsigned int v, int c) { return (v << c) | (v >> (32 - c)); };
+  for (size_t blk = 0; blk * 16 < n; ++blk) {
+    unsigned int in[16] = {0x61707865u, 0x3320646eu, 0x79622d32u, 0x6b206574u, key[0], key[1], key[2], key[3],
+                           key[4], key[5], key[6], key[7], (unsigned int)blk, (unsigned int)(blk >> 32), 0u, 0u};
+    unsigned int x[16];
+    std::memcpy(x, in, sizeof x);
+    for (int r = 0; r < 8; r += 2) {
+      auto qr = [&](int a, int b, int c, int d) {
+        x[a] += x[b]; x[d] = rotl(x[d] ^ x[a], 16);
+        x[c] += x[d]; x[b] = rotl(x[b] ^ x[c], 12);
+        x[a] += x[b]; x[d] = rotl(x[d] ^ x[a], 8);
+        x[c] += x[d]; x[b] = rotl(x[b] ^ x[c], 7);
+      };
+      qr(0, 4, 8, 12); qr(1, 5, 9, 13); qr(2, 6, 10, 14); qr(3, 7, 11, 15);
+      qr(0, 5, 10, 15); qr(1, 6, 11, 12); qr(2, 7, 8, 13); qr(3, 4, 9, 14);
+    }
+    for (int w = 0; w < 16 && blk * 16 + w < n; ++w) out[blk * 16 + w] = x[w] + in[w];
+  }
+}
+
+template <class T>
+T* upload(const std::vector<T>& v) {
+  T* d = nullptr;
+  if (v.empty()) return nullptr;
+  hip_check(hipMalloc(&d, v.size() * sizeof(T)), "hipMalloc");
+  hip_check(hipMemcpy(d, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy");
+  return d;
+}
+
+}  // namespace
+
+struct sdesim_plan {
+  CompiledKernel kernel;
+  std::string source;
+  SimArgs args{};
+  sdesim_options opt{};
+  int P = 0, M = 0, KQ = 0;
+  int block = 256, ppl = 1;
+  long long grid = 0;
+  std::vector<void*> owned;
+  double* partials = nullptr;
+  double* moments = nullptr;
+  double* shifts = nullptr;
+  ~sdesim_plan() {
+    for (void* p : owned) (void)hipFree(p);
+  }
+  template <class T>
+  T* own(T* p) {
+    if (p) owned.push_back((void*)p);
+    return p;
+  }
+};
+
+extern "C" {
+
+void sdesim_default_options(sdesim_options* o) {
+  std::memset(o, 0, sizeof *o);
+  o->scheme = SDESIM_EULER;
+  o->rng = SDESIM_PSEUDO;
+  o->scramble = SDESIM_SCRAMBLE_SHIFT;
+  o->precision = SDESIM_F64;
+  o->compat = SDESIM_COMPAT_REFERENCE;
+  o->outputs = SDESIM_OUT_FILTRATION;
+  o->sobol_first = -1;
+}
+
+int sdesim_parse_equations(const char* const* equations, int32_t n, sdesim_universe** out) {
+  return guarded([&] {
+    if (!out) fail(SDESIM_EVALUE, "null output handle");
+    std::vector<std::string> eqs;
+    for (int32_t i = 0; i < n; ++i) eqs.emplace_back(equations[i] ? equations[i] : "");
+    std::unique_ptr<sdesim_universe> u(new sdesim_universe());
+    u->u = sdesim::parse_equations(eqs);
+    *out = u.release();
+  });
+}
+
+void sdesim_free_universe(sdesim_universe* u) { delete u; }
+
+int32_t sdesim_num_processes(const sdesim_universe* u) { return u ? u->u.P() : 0; }
+int32_t sdesim_num_increments(const sdesim_universe* u) { return u ? u->u.M() : 0; }
+const char* sdesim_process_name(const sdesim_universe* u, int32_t p) {
+  if (!u || p < 0 || p >= u->u.P()) return nullptr;
+  return u->u.processes[p].name.c_str();
+}
+int32_t sdesim_process_is_algebraic(const sdesim_universe* u, int32_t p) {
+  if (!u || p < 0 || p >= u->u.P()) return -1;
+  return u->u.processes[p].levy ? 0 : 1;
+}
+int32_t sdesim_process_index(const sdesim_universe* u, const char* name) {
+  if (!u || !name) return -1;
+  auto it = u->u.registry.find(name);
+  return it == u->u.registry.end() ? -1 : it->second;
+}
+
+int sdesim_plan_create(const sdesim_universe* uh, const double* times, int32_t T, const double* x0,
+                       int64_t S, const sdesim_options* opt_in, const char* payoff, sdesim_plan** out) {
+  return guarded([&] {
+    if (!uh || !out) fail(SDESIM_EVALUE, "null handle");
+    if (S <= 0) fail(SDESIM_EVALUE, "scenarios must be a positive integer");
+    if (T <= 0 || !times) fail(SDESIM_EPANIC, "time_steps must not be empty");
+    sdesim_options opt;
+    if (opt_in) opt = *opt_in; else sdesim_default_options(&opt);
+    const sdesim::Universe& U = uh->u;
+    {
+      std::set<double> seen;
+      for (int32_t i = 0; i < T; ++i) {
+        if (std::isnan(times[i])) fail(SDESIM_EVALUE, "time_steps must not contain NaN");
+        if (!seen.insert(times[i]).second) fail(SDESIM_EVALUE, "time_steps must not contain duplicate values");
+      }
+    }
+    const int N = T - 1, P = U.P(), M = U.M();
+    if (opt.scheme != SDESIM_EULER && opt.scheme != SDESIM_RUNGE_KUTTA) fail(SDESIM_EPANIC, "not implemented");
+    if (opt.rng == SDESIM_SOBOL && (long long)N * M > sdesim_jk_dims)
+      fail(SDESIM_EPANIC, "Sobol dimension (time steps x increments) exceeds " + std::to_string(sdesim_jk_dims));
+
+    std::unique_ptr<sdesim_plan> plan(new sdesim_plan());
+    plan->opt = opt;
+    plan->P = P;
+    plan->M = M;
+    plan->args.S = S;
+    plan->args.T = T;
+    plan->args.N = N;
+    if (P == 0) {  // no processes: every scenario has an empty filtration; nothing to launch
+      plan->opt.outputs = 0;
+      *out = plan.release();
+      return;
+    }
+    const bool has_payoff = payoff && *payoff;
+    plan->KQ = P + (has_payoff ? 1 : 0);
+    plan->block = opt.block_size > 0 ? opt.block_size : 256;
+    if (plan->block % 64 != 0 || plan->block > 1024) fail(SDESIM_EVALUE, "block_size must be a multiple of 64 up to 1024");
+    plan->ppl = opt.paths_per_lane > 0 ? opt.paths_per_lane : 1;
+    if (plan->ppl > 8) fail(SDESIM_EVALUE, "paths_per_lane must be at most 8");
+
+    int rng_kind;
+    if (M == 0) rng_kind = SDE_RNG_NONE;
+    else if (opt.rng == SDESIM_PSEUDO) rng_kind = SDE_RNG_PSEUDO;
+    else if (opt.rng == SDESIM_SOBOL) rng_kind = opt.scramble == SDESIM_SCRAMBLE_XOR ? SDE_RNG_SOBOL_XOR : SDE_RNG_SOBOL_SHIFT;
+    else if (opt.rng == SDESIM_INJECTED) rng_kind = SDE_RNG_INJECTED;
+    else rng_kind = SDE_RNG_PSEUDO;  // the reference treats any other string as "pseudo"
+
+    sdesim::CodegenOptions co;
+    co.scheme = opt.scheme;
+    co.compat = opt.compat;
+    co.f32 = opt.precision == SDESIM_F32;
+    co.rng = rng_kind;
+    co.out = opt.outputs & (SDESIM_OUT_FILTRATION | SDESIM_OUT_TERMINAL);
+    co.ppl = plan->ppl;
+    co.block = plan->block;
+    co.steps = N > 0;
+    co.payoff = has_payoff ? payoff : "";
+    plan->source = sdesim::generate_kernel_source(U, co);
+    plan->kernel = compile_kernel(plan->source);
+
+    // Time grid in the reference's arithmetic: dt = t[n+1] - t[n], sqrt(dt) (increment.rs:38-41,75-79).
+    std::vector<double> tv(times, times + T), dts(N), sdts(N);
+    std::vector<float> tvf(T), dtsf(N), sdtsf(N);
+    for (int n = 0; n < N; ++n) { dts[n] = tv[n + 1] - tv[n]; sdts[n] = std::sqrt(dts[n]); }
+    for (int i = 0; i < T; ++i) tvf[i] = (float)tv[i];
+    for (int n = 0; n < N; ++n) { dtsf[n] = (float)dts[n]; sdtsf[n] = (float)sdts[n]; }
+    std::vector<double> x0v(x0 ? x0 : nullptr, x0 ? x0 + P : nullptr);
+    if (!x0) x0v.assign(P, 0.0);
+    if (x0v.empty()) x0v.push_back(0.0);
+
+    SimArgs& a = plan->args;
+    a.times = plan->own(upload(tv));
+    a.dts = plan->own(upload(dts));
+    a.sdts = plan->own(upload(sdts));
+    a.times_f = plan->own(upload(tvf));
+    a.dts_f = plan->own(upload(dtsf));
+    a.sdts_f = plan->own(upload(sdtsf));
+    a.x0 = plan->own(upload(x0v));
+    a.S = S;
+    a.seed = opt.seed;
+    a.N = N;
+    a.T = T;
+    a.nslices = 1;
+    if (rng_kind == SDE_RNG_SOBOL_SHIFT || rng_kind == SDE_RNG_SOBOL_XOR) {
+      long long first = opt.sobol_first >= 0 ? opt.sobol_first : (rng_kind == SDE_RNG_SOBOL_XOR ? 0 : 5);
+      a.sobol_first = first;
+      unsigned long long last = (unsigned long long)(first + S - 1);
+      unsigned long long g = last | (last >> 1);
+      int bits = 0;
+      while (bits < 64 && (g >> bits) != 0ull) ++bits;
+      if (bits > 32) fail(SDESIM_EPANIC, "Sobol sequence exhausted");
+      int nslices = 1;
+      while (6 + 8 * (nslices - 1) < bits) ++nslices;
+      SobolTables st = sobol_tables(std::max(1, N * M), nslices);
+      a.sobol_tab = st.dev;
+      a.nslices = nslices;
+      if (rng_kind == SDE_RNG_SOBOL_XOR) {
+        std::vector<unsigned int> masks;
+        chacha8_words(opt.scramble_seed, (size_t)std::max(1, N * M), masks);
+        a.xor_mask = plan->own(upload(masks));
+      }
+    }
+    plan->grid = (S + (long long)plan->block * plan->ppl - 1) / ((long long)plan->block * plan->ppl);
+    if (plan->grid > 0x7fffffffLL) fail(SDESIM_EVALUE, "too many scenarios for one launch");
+    if (opt.outputs & SDESIM_OUT_TERMINAL) {
+      hip_check(hipMalloc(&plan->partials, (size_t)plan->grid * 2 * plan->KQ * sizeof(double)), "hipMalloc(partials)");
+      plan->own(plan->partials);
+      hip_check(hipMalloc(&plan->moments, (size_t)2 * plan->KQ * sizeof(double)), "hipMalloc(moments)");
+      plan->own(plan->moments);
+      std::vector<double> shifts(plan->KQ, 0.0);
+      for (int p = 0; p < P; ++p) shifts[p] = x0v[p];
+      plan->shifts = plan->own(upload(shifts));
+      a.partials = plan->partials;
+    }
+    *out = plan.release();
+  });
+}
+
+void sdesim_plan_free(sdesim_plan* plan) { delete plan; }
+
+int64_t sdesim_plan_filtration_bytes(const sdesim_plan* plan) {
+  if (!plan) return 0;
+  const long long b = plan->opt.precision == SDESIM_F32 ? 4 : 8;
+  return (int64_t)plan->args.T * plan->P * plan->args.S * b;
+}
+
+const char* sdesim_plan_source(const sdesim_plan* plan) { return plan ? plan->source.c_str() : nullptr; }
+
+int sdesim_plan_run(sdesim_plan* plan, void* d_filtration, const double* d_injected, void* stream) {
+  return guarded([&] {
+    if (!plan) fail(SDESIM_EVALUE, "null plan");
+    if (plan->P == 0) return;
+    SimArgs a = plan->args;
+    if ((plan->opt.outputs & SDESIM_OUT_FILTRATION) && !d_filtration && plan->P > 0)
+      fail(SDESIM_EVALUE, "filtration output requested but no buffer given");
+    a.filtration = d_filtration;
+    if (plan->opt.rng == SDESIM_INJECTED && plan->M > 0) {
+      if (!d_injected) fail(SDESIM_EVALUE, "injected uniforms requested but no buffer given");
+      a.injected = d_injected;
+    }
+    void* params[] = {&a};
+    hip_check(hipModuleLaunchKernel(plan->kernel.function, (unsigned)plan->grid, 1, 1, (unsigned)plan->block, 1, 1, 0,
+                                    (hipStream_t)stream, params, nullptr),
+              "hipModuleLaunchKernel");
+    if (plan->opt.outputs & SDESIM_OUT_TERMINAL) {
+      hip_check(sdesim_launch_finalize(plan->partials, (int)plan->grid, plan->KQ, a.S, plan->shifts, plan->moments,
+                                       (hipStream_t)stream),
+                "finalize");
+    }
+  });
+}
+
+int sdesim_plan_moments(sdesim_plan* plan, double* h_out, void* stream) {
+  return guarded([&] {
+    if (!plan || !h_out) fail(SDESIM_EVALUE, "null argument");
+    if (!(plan->opt.outputs & SDESIM_OUT_TERMINAL)) fail(SDESIM_EVALUE, "plan has no terminal output");
+    hip_check(hipMemcpyAsync(h_out, plan->moments, sizeof(double) * 2 * plan->KQ, hipMemcpyDeviceToHost,
+                             (hipStream_t)stream),
+              "hipMemcpyAsync(moments)");
+    hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+  });
+}
+
+int sdesim_filtration_to_long(const void* d_filtration, int32_t precision, int64_t S, int32_t T, int32_t P,
+                              double* d_long, void* stream) {
+  return guarded([&] {
+    if (S <= 0 || T <= 0 || P < 0) fail(SDESIM_EVALUE, "bad shape");
+    if (P == 0) return;
+    hip_check(sdesim_launch_to_long(d_filtration, precision == SDESIM_F32 ? 1 : 0, S, T * P, d_long,
+                                    (hipStream_t)stream),
+              "to_long");
+  });
+}
+
+int sdesim_simulate_long(const sdesim_universe* u, const double* times, int32_t T, const char* const* init_names,
+                         const double* init_values, int32_t n_init, int64_t S, const sdesim_options* opt_in,
+                         double* h_values, void* stream) {
+  sdesim_plan* plan = nullptr;
+  void* d_filt = nullptr;
+  double* d_long = nullptr;
+  int rc = guarded([&] {
+    if (!u) fail(SDESIM_EVALUE, "null universe");
+    if (S <= 0) fail(SDESIM_EVALUE, "scenarios must be a positive integer");
+    const int P = u->u.P();
+    std::vector<double> x0(P, 0.0);
+    for (int32_t i = 0; i < n_init; ++i) {
+      auto it = u->u.registry.find(init_names[i]);
+      if (it != u->u.registry.end()) x0[it->second] = init_values[i];
+    }
+    sdesim_options opt;
+    if (opt_in) opt = *opt_in; else sdesim_default_options(&opt);
+    opt.outputs = SDESIM_OUT_FILTRATION;
+    int r = sdesim_plan_create(u, times, T, x0.data(), S, &opt, nullptr, &plan);
+    if (r != SDESIM_OK) fail(r, g_last_error);
+    if (P == 0) return;
+    const size_t vals = (size_t)T * P * S;
+    size_t free_b = 0, total_b = 0;
+    hip_check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    const size_t need = (size_t)sdesim_plan_filtration_bytes(plan) + vals * sizeof(double);
+    if (need + (64u << 20) > free_b)
+      fail(SDESIM_ERUNTIME, "output of " + std::to_string(need) + " bytes does not fit in free device memory");
+    hip_check(hipMalloc(&d_filt, (size_t)sdesim_plan_filtration_bytes(plan)), "hipMalloc(filtration)");
+    hip_check(hipMalloc(&d_long, vals * sizeof(double)), "hipMalloc(long)");
+    r = sdesim_plan_run(plan, d_filt, nullptr, stream);
+    if (r != SDESIM_OK) fail(r, g_last_error);
+    r = sdesim_filtration_to_long(d_filt, opt.precision, S, T, P, d_long, stream);
+    if (r != SDESIM_OK) fail(r, g_last_error);
+    hip_check(hipMemcpyAsync(h_values, d_long, vals * sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)stream),
+              "hipMemcpyAsync(values)");
+    hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+  });
+  if (d_filt) (void)hipFree(d_filt);
+  if (d_long) (void)hipFree(d_long);
+  sdesim_plan_free(plan);
+  return rc;
+}
+
+int sdesim_generate_source(const sdesim_universe* u, int32_t n_times, const sdesim_options* opt_in, const char* payoff,
+                           char* buf, int64_t buf_len, int64_t* needed) {
+  return guarded([&] {
+    if (!u) fail(SDESIM_EVALUE, "null universe");
+    sdesim_options opt;
+    if (opt_in) opt = *opt_in; else sdesim_default_options(&opt);
+    sdesim::CodegenOptions co;
+    co.scheme = opt.scheme;
+    co.compat = opt.compat;
+    co.f32 = opt.precision == SDESIM_F32;
+    co.rng = u->u.M() == 0 ? SDE_RNG_NONE
+             : opt.rng == SDESIM_SOBOL ? (opt.scramble == SDESIM_SCRAMBLE_XOR ? SDE_RNG_SOBOL_XOR : SDE_RNG_SOBOL_SHIFT)
+             : opt.rng == SDESIM_INJECTED ? SDE_RNG_INJECTED : SDE_RNG_PSEUDO;
+    co.out = opt.outputs & (SDESIM_OUT_FILTRATION | SDESIM_OUT_TERMINAL);
+    co.ppl = opt.paths_per_lane > 0 ? opt.paths_per_lane : 1;
+    co.block = opt.block_size > 0 ? opt.block_size : 256;
+    co.steps = n_times > 1;
+    co.payoff = payoff ? payoff : "";
+    std::string src = sdesim::generate_kernel_source(u->u, co);
+    if (needed) *needed = (int64_t)src.size() + 1;
+    if (buf && buf_len > 0) std::snprintf(buf, (size_t)buf_len, "%s", src.c_str());
+  });
+}
+
+int sdesim_compile_source(const char* source, int64_t* code_bytes) {
+  return guarded([&] {
+    std::vector<char> code = rtc_compile(source ? source : "");
+    if (code_bytes) *code_bytes = (int64_t)code.size();
+  });
+}
+
+const char* sdesim_last_error(void) { return g_last_error.c_str(); }
+int32_t sdesim_version(void) { return 100; }
+
+}  // extern "C"
