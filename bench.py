@@ -1,0 +1,164 @@
+"""Benchmark: Heston RQMC path simulation on one MI355X.
+
+Workload (the reference's headline configuration): Heston (S, v) with S0=100, v0=0.04,
+kappa=2, theta=0.04, xi=0.5, rho=-0.7, r=0.02, T=1, written in the reference's equation DSL;
+"Runge-Kutta first order" scheme with the reference's cache semantics; 1024 steps; 2^20 paths;
+Sobol points of dimension 2048 with a digital XOR scramble from seed 7.
+
+One step = one launch of the fused kernel over all 2^20 paths x 1024 steps, inputs resident in
+HBM. Headline: FP32 state, full filtration written to HBM (2^20 x 1025 x 2 x 4 B = 8.6 GB).
+Reported beside it, in the same run: the plain FP64 kernel (full filtration, 17.2 GB), the
+terminal-only modes (European call payoff max(S - 100, 0), FP64 estimator sums) in FP32 and FP64
+with the FP32 estimator error against FP64, and the oracle (the C++ restatement of the
+reference's CPU simulator) timed on the host's cores on a bounded sample.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "sde-sim-rs_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HESTON = [
+    "dS = (0.02 * S) * dt + (max(v, 0)^0.5 * S) * dW1",
+    "dv = (2.0 * (0.04 - max(v, 0))) * dt + (-0.7 * 0.5 * max(v, 0)^0.5) * dW1"
+    " + (0.7141428428542850 * 0.5 * max(v, 0)^0.5) * dW2",
+]
+INIT = {"S": 100.0, "v": 0.04}
+PAYOFF = "max(S - 100, 0)"
+HBM_PEAK_GBS = 8000.0
+
+
+def _time_plan(plan, out, steps, warmup):
+    """Returns (wall ms per step, mean kernel ms from per-launch HIP events)."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        plan.run(out, stream=stream)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for a, b in ev:
+        a.record(stream)
+        plan.run(out, stream=stream)
+        b.record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) * 1e3 / steps
+    kern = sum(a.elapsed_time(b) for a, b in ev) / steps
+    return wall, kern
+
+
+def cpu_baseline(paths, steps_n, threads):
+    import numpy as np
+
+    from oracle import oracle as O
+
+    times = np.linspace(0.0, 1.0, steps_n + 1)
+    O.simulate(HESTON, times, 64, INIT, scheme="runge-kutta", rng="sobol", scramble="xor", scramble_seed=7,
+               threads=threads)
+    t0 = time.perf_counter()
+    O.simulate(HESTON, times, paths, INIT, scheme="runge-kutta", rng="sobol", scramble="xor", scramble_seed=7,
+               threads=threads)
+    dt = time.perf_counter() - t0
+    return paths * steps_n / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--paths", type=int, default=1 << 20)
+    ap.add_argument("--nsteps", type=int, default=1024)
+    ap.add_argument("--ppl", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--cpu-paths", type=int, default=1 << 15)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="", help="comma list of legs to run: f32,f64,term (profiling)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: paths are independent and the multi-GPU form "
+              "(shard scenarios, one all-reduce of the estimator partials) is not built.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import sde_sim_rs
+
+    S, N = args.paths, args.nsteps
+    times = np.linspace(0.0, 1.0, N + 1)
+    legs = set(args.only.split(",")) if args.only else {"f32", "f64", "term"}
+    common = dict(seed=0, scramble="xor", scramble_seed=7, paths_per_lane=args.ppl, block_size=args.block)
+    uni = sde_sim_rs.Universe(HESTON)
+    res = {}
+
+    if "f32" in legs:
+        p32 = sde_sim_rs.Plan(uni, times, S, INIT, "sobol", "runge-kutta", precision="f32", **common)
+        f32 = p32.new_filtration()
+        res["f32"] = _time_plan(p32, f32, args.steps, args.warmup)
+        del f32
+    if "f64" in legs:
+        p64 = sde_sim_rs.Plan(uni, times, S, INIT, "sobol", "runge-kutta", precision="f64", **common)
+        f64 = p64.new_filtration()
+        res["f64"] = _time_plan(p64, f64, args.steps, args.warmup)
+        del f64
+    term = {}
+    if "term" in legs:
+        for prec in ("f32", "f64"):
+            pt = sde_sim_rs.Plan(uni, times, S, INIT, "sobol", "runge-kutta", precision=prec, outputs=("terminal",),
+                                 payoff=PAYOFF, **common)
+            w, k = _time_plan(pt, None, args.steps, args.warmup)
+            m = pt.moments()
+            term[prec] = {"ms": k, "path_steps_per_s": S * N / (k * 1e-3), "payoff_mean": m["payoff"][0],
+                          "payoff_var": m["payoff"][1], "S_mean": m["S"][0]}
+        torch.cuda.synchronize()
+
+    path_steps = S * N
+    line = {"metric": "path-steps/s on 1 MI355X, FP32 vs FP64 plain at equal error; HBM GB/s %peak",
+            "unit": "path-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+            "data": "synthetic (Sobol-XOR RQMC, seed 7)",
+            "config": {"workload": "heston_rk_rqmc_full_filtration", "paths": S, "steps": N, "processes": 2,
+                       "sobol_dims": 2 * N, "scramble": "xor", "scramble_seed": 7, "scheme": "runge-kutta",
+                       "state": "f32", "estimator_sums": "f64", "parallelism": "single GPU"}}
+    if "f32" in res:
+        wall, kern = res["f32"]
+        bytes_alg = (N + 1) * 2 * S * 4
+        line.update({"value": path_steps / (wall * 1e-3), "ms_per_step": wall})
+        ach = bytes_alg / (kern * 1e-3) / 1e9
+        line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                            "kernel": "sdesim_kernel (f32, full filtration)", "kernel_ms": kern,
+                            "algorithmic_bytes": bytes_alg}
+    if "f64" in res:
+        wall, kern = res["f64"]
+        bytes_alg = (N + 1) * 2 * S * 8
+        line["fp64_plain"] = {"path_steps_per_s": path_steps / (wall * 1e-3), "ms_per_step": wall, "kernel_ms": kern,
+                              "hbm_gbs": bytes_alg / (kern * 1e-3) / 1e9}
+        if "f32" in res:
+            line["fp32_over_fp64"] = res["f64"][0] / res["f32"][0]
+    if term:
+        e = abs(term["f32"]["payoff_mean"] - term["f64"]["payoff_mean"]) / abs(term["f64"]["payoff_mean"])
+        term["fp32_payoff_rel_err_vs_fp64"] = e
+        term["fp32_over_fp64"] = term["f64"]["ms"] / term["f32"]["ms"]
+        line["terminal_only"] = term
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        v, dt = cpu_baseline(args.cpu_paths, N, threads)
+        line["cpu_baseline"] = {"value": v, "unit": "path-steps/s", "cores": threads, "kind": "port",
+                                "sample": "same workload, %d paths x %d steps, oracle restatement of the "
+                                          "reference simulator (f64, full filtration), %.1f s" % (args.cpu_paths, N, dt)}
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

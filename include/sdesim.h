@@ -126,6 +126,17 @@ int sdesim_generate_source(const sdesim_universe* u, int32_t n_times, const sdes
                            char* buf, int64_t buf_len, int64_t* needed);
 int sdesim_compile_source(const char* source, int64_t* code_bytes);
 
+/* Host evaluation of one coefficient expression with the kernels' semantics (the value
+ * Function::eval, src/func.rs:32-42, returns for that namespace). An undefined name returns
+ * SDESIM_EPANIC, as the reference's unwrap does. */
+int sdesim_eval_expression(const char* expr, const char* const* names, const double* values, int32_t n,
+                           double* out);
+
+/* Sobol points in Gray-code order from the same sliced tables the kernels read: out[i][d] for
+ * point index first + i, dims dimensions (the reference's SobolEngine, src/rng/sobol.rs:10-26,
+ * before its f64 conversion x * 2^-32). */
+int sdesim_sobol_points(int32_t dims, uint64_t first, int64_t n, uint32_t* out);
+
 const char* sdesim_last_error(void);
 int32_t sdesim_version(void);
 

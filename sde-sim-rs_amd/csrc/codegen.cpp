@@ -164,8 +164,8 @@ class StepEmitter {
       case IncKind::Jump: {
         std::string lam = eval(*t.lambda, false);
         std::string name = "j" + std::to_string(tmp_++);
-        line("const " + R_ + " " + name + " = (" + R_ + ")sde_inv_poisson(u[" + std::to_string(t.index) +
-             "], (double)(" + lam + " * dt));");
+        line("const " + R_ + " " + name + " = (" + R_ + ")sde_inv_poisson(sde_uval(u[" + std::to_string(t.index) +
+             "]), (double)(" + lam + " * dt));");
         return name;
       }
     }
@@ -203,7 +203,7 @@ class StepEmitter {
   void rk() {
     if (u_.M() == 0) throw Panic("RNG Index 0 out of bounds (max 0)");
     if (o_.compat == 1) refresh(false);
-    line("const " + R_ + " sk = (u[0] > 0.5) ? " + R_ + "(1) : " + R_ + "(-1);");
+    line("const " + R_ + " sk = sde_gt_half(u[0]) ? " + R_ + "(1) : " + R_ + "(-1);");
     const int P = u_.P();
     std::vector<std::vector<std::string>> d(P);
     for (int p = 0; p < P; ++p) {
@@ -211,6 +211,11 @@ class StepEmitter {
       if (!pd.levy) continue;
       for (const Term& t : pd.terms) d[p].push_back(sample(t));
     }
+    // Sums start at their first term rather than at 0 + first term (the reference's
+    // `k1 = 0.0; k1 += c * d`): the values are identical except that an exact -0.0 term stays
+    // -0.0, and the device can fuse every product into an FMA. (c * sk) * sdt is evaluated as
+    // c * (sk * sdt), exactly equal because sk = +-1.
+    line("const " + R_ + " sks = sk * sdt;");
     for (int p = 0; p < P; ++p) {
       const ProcessDef& pd = u_.processes[p];
       if (!pd.levy) continue;
@@ -218,7 +223,8 @@ class StepEmitter {
       line(R_ + " " + k1 + " = " + R_ + "(0);");
       for (size_t k = 0; k < pd.terms.size(); ++k) {
         std::string c = eval(*pd.terms[k].coef, false);
-        line(k1 + " = " + k1 + " + " + c + " * " + d[p][k] + ";");
+        if (k == 0) line(k1 + " = " + c + " * " + d[p][k] + ";");
+        else line(k1 + " = " + k1 + " + " + c + " * " + d[p][k] + ";");
       }
     }
     for (int p = 0; p < P; ++p) {
@@ -226,10 +232,13 @@ class StepEmitter {
       if (!pd.levy) continue;
       std::string pe = "pe" + std::to_string(p);
       line(R_ + " " + pe + " = " + R_ + "(0);");
+      bool first = true;
       for (const Term& t : pd.terms) {
         if (t.kind != IncKind::Wiener) continue;
         std::string c = eval(*t.coef, false);
-        line(pe + " = " + pe + " + " + c + " * sk * sdt;");
+        if (first) line(pe + " = " + c + " * sks;");
+        else line(pe + " = " + pe + " + " + c + " * sks;");
+        first = false;
       }
       line(y(p) + " = X[" + std::to_string(p) + "] + k1_" + std::to_string(p) + " + " + pe + ";");
     }
@@ -240,7 +249,8 @@ class StepEmitter {
       line(R_ + " " + k2 + " = " + R_ + "(0);");
       for (size_t k = 0; k < pd.terms.size(); ++k) {
         std::string c = eval(*pd.terms[k].coef, true);
-        line(k2 + " = " + k2 + " + " + c + " * " + d[p][k] + ";");
+        if (k == 0) line(k2 + " = " + c + " * " + d[p][k] + ";");
+        else line(k2 + " = " + k2 + " + " + c + " * " + d[p][k] + ";");
       }
     }
     for (int p : u_.levy)
@@ -302,8 +312,9 @@ std::string generate_kernel_source(const Universe& u, const CodegenOptions& o) {
   s << "  __device__ static __forceinline__ void prologue(real (&X)[P], real (&C)[P], real (&K)[NK], real tc) {\n";
   s << pro;
   s << "  }\n";
+  s << "  template <class UT>\n";
   s << "  __device__ static __forceinline__ void step(real (&X)[P], real (&C)[P], real (&K)[NK], real tc, real tn,\n";
-  s << "                                              real dt, real sdt, const double (&u)[" << (M > 0 ? M : 1) << "]) {\n";
+  s << "                                              real dt, real sdt, const UT (&u)[" << (M > 0 ? M : 1) << "]) {\n";
   s << body << tail.str();
   s << "  }\n";
   s << "  __device__ static __forceinline__ real payoff(const real (&X)[P], real tt) {\n";
@@ -323,10 +334,18 @@ std::string generate_kernel_source(const Universe& u, const CodegenOptions& o) {
   }
   s << "  }\n";
   s << "};\n\n";
-  s << "extern \"C\" __global__ void __launch_bounds__(" << o.block << ") sdesim_kernel(SimArgs a) {\n";
-  s << "  sde_path_kernel<SdeModel, " << o.rng << ", " << o.out << ", " << o.ppl << ">(a);\n";
+  s << "extern \"C\" __global__ void __launch_bounds__(" << o.block << ") " << o.name << "(SimArgs a) {\n";
+  s << "  sde_path_kernel<SdeModel, " << o.rng << ", " << o.out << ", " << o.ppl << ", " << o.nsl << ", "
+    << (o.uniform_grid ? "true" : "false") << ">(a);\n";
   s << "}\n";
   return s.str();
+}
+
+std::string kernel_name(const CodegenOptions& o) {
+  static const char* rng[] = {"pseudo", "sobolshift", "sobolxor", "injected", "none"};
+  std::string out = o.out == 1 ? "filt" : o.out == 2 ? "term" : "filt_term";
+  return std::string("sdesim_") + (o.scheme == 0 ? "em" : "rk") + (o.f32 ? "_f32_" : "_f64_") +
+         rng[o.rng >= 0 && o.rng <= 4 ? o.rng : 4] + "_" + out;
 }
 
 }  // namespace sdesim

@@ -15,8 +15,11 @@ struct CodegenOptions {
   int out = 1;          // SDE_OUT_* bit mask
   int ppl = 1;          // paths per lane
   int block = 256;      // threads per block
+  int nsl = 1;          // Sobol table slices (Gray-code bits covered: 6 + 8 (nsl - 1))
+  bool uniform_grid = false;  // every dt and sqrt(dt) is bitwise equal in the kernel's precision
   bool steps = true;    // false when the time grid has a single point (nothing is evaluated)
   std::string payoff;   // expression over process names and t at the last time point
+  std::string name = "sdesim_kernel";  // entry point name
 };
 
 struct UndefinedName : std::runtime_error {
@@ -28,9 +31,12 @@ struct Panic : std::runtime_error {
   using std::runtime_error::runtime_error;
 };
 
-// Returns the complete kernel source (entry point "sdesim_kernel"). Throws ParseError for a bad
+// Returns the complete kernel source (entry point `o.name`). Throws ParseError for a bad
 // payoff, UndefinedName when an evaluated expression names an unknown variable, and
 // Panic for the reference's run-time panics (RK with no stochastic increment).
 std::string generate_kernel_source(const Universe& u, const CodegenOptions& o);
+
+// Descriptive entry point name: sdesim_<em|rk>_<f32|f64>_<rng>_<filt|term|filt_term>.
+std::string kernel_name(const CodegenOptions& o);
 
 }  // namespace sdesim

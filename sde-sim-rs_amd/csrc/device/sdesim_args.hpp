@@ -22,13 +22,15 @@ struct SimArgs {
   const double* injected;    // [N * M][S] injected uniforms (test hook)
   void* filtration;          // [T][P][S] real, or null
   double* partials;          // [gridDim.x][2 * KQ] (sum, sum of squares) of shifted values, or null
+  double dt_u, sdt_u;        // the step when every dt (and sqrt dt) is bitwise equal (f64 kernels)
+  float dtf_u, sdtf_u;       // the same for f32 kernels
   long long S;               // paths
   sde_u64 seed;              // path s uses seed + s (wrapping)
   long long sobol_first;     // Sobol point index of path 0
   int N;                     // steps
   int T;                     // time points
   int nslices;               // Sobol table slices
-  int pad_;
+  int sobol_tab_words;       // entries in sobol_tab (bounds of its buffer resource)
 };
 
 #define SDE_RNG_PSEUDO 0

@@ -75,17 +75,15 @@ __device__ __forceinline__ double sde_u01_from_words(sde_u32 lo, sde_u32 hi) {
 // ------------------------------------------------------------------------------------------
 // Sobol
 // ------------------------------------------------------------------------------------------
-// Coordinate `d` of the point whose Gray code is `g`: XOR of one entry per table slice
-// (slice 0 covers Gray bits 0-5, slice k >= 1 covers bits 6+8(k-1) .. 13+8(k-1)).
-__device__ __forceinline__ sde_u32 sde_sobol_coord(const sde_u32* __restrict__ tab, int nslices, int d, sde_u64 g) {
-  const int stride = 64 + 256 * (nslices - 1);
-  const sde_u32* t = tab + (long long)d * stride;
-  sde_u32 x = t[g & 63u];
+// The table row of a dimension holds one 64-entry slice for Gray-code bits 0-5 and one
+// 256-entry slice per further byte (bits 6+8(k-1) .. 13+8(k-1)); coordinate d of the point with
+// Gray code g is the XOR of one entry per slice. sde_sobol_offsets gives a path's entry index
+// within each slice row (fixed for the whole path).
+template <int NSL>
+__device__ __forceinline__ void sde_sobol_offsets(sde_u64 g, sde_u32 (&off)[NSL]) {
+  off[0] = (sde_u32)(g & 63u);
 #pragma unroll
-  for (int k = 1; k < 5; ++k) {
-    if (k < nslices) x ^= t[64 + 256 * (k - 1) + ((g >> (6 + 8 * (k - 1))) & 255u)];
-  }
-  return x;
+  for (int k = 1; k < NSL; ++k) off[k] = 64u + 256u * (k - 1) + (sde_u32)((g >> (6 + 8 * (k - 1))) & 255u);
 }
 
 // ------------------------------------------------------------------------------------------
@@ -100,19 +98,27 @@ __device__ __forceinline__ double sde_inv_normal(double p) {
   return (p < 0.5) ? -x : x;
 }
 
+// f64 evaluation of |z| for a small q = min(p, 1 - p). The empty volatile asm makes the block
+// non-speculatable, so callers keep it behind a branch that waves skip when no lane needs it
+// (probability ~6e-7 per draw) instead of if-converting it into every step.
+__device__ __forceinline__ float sde_inv_normal_tail(double q) {
+  asm volatile("");
+  const double t = sqrt(-2.0 * log(q));
+  return (float)(t - ((0.010328 * t + 0.802853) * t + 2.515517) /
+                         (((0.001308 * t + 0.189269) * t + 1.432788) * t + 1.0));
+}
+
 // The same rational approximation in f32. q = min(p, 1 - p) is formed in f64 (exact for a
 // 53-bit uniform) so the tails keep their resolution; q < 2.87e-7 (|z| > 5) takes the f64 path.
 __device__ __forceinline__ float sde_inv_normal_f32(double p) {
   const double qd = (p < 0.5) ? p : 1.0 - p;
   float x;
   if (qd < 2.87e-7) {
-    const double t = sqrt(-2.0 * log(qd));
-    x = (float)(t - ((0.010328 * t + 0.802853) * t + 2.515517) /
-                        (((0.001308 * t + 0.189269) * t + 1.432788) * t + 1.0));
+    x = sde_inv_normal_tail(qd);
   } else {
     const float q = (float)qd;
-    // -2 ln q = -2 ln(2) log2(q); v_log_f32 is a base-2 logarithm.
-    const float t = __builtin_sqrtf(-1.3862943611198906f * __log2f(q));
+    // -2 ln q = -2 ln(2) log2(q); v_log_f32 is a base-2 logarithm (q >= 2.87e-7 is normal).
+    const float t = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(q));
     const float num = (0.010328f * t + 0.802853f) * t + 2.515517f;
     const float den = ((0.001308f * t + 0.189269f) * t + 1.432788f) * t + 1.0f;
     x = t - num * __builtin_amdgcn_rcpf(den);
@@ -120,9 +126,41 @@ __device__ __forceinline__ float sde_inv_normal_f32(double p) {
   return (p < 0.5) ? -x : x;
 }
 
+// A uniform from the digital-XOR-scrambled Sobol stream, kept as its 32-bit integer y; its
+// value is (y + 0.5) * 2^-32, strictly inside (0, 1).
+struct SdeUnifX {
+  sde_u32 y;
+};
+
+__device__ __forceinline__ double sde_uval(double u) { return u; }
+__device__ __forceinline__ double sde_uval(SdeUnifX u) { return ((double)u.y + 0.5) * 0x1.0p-32; }
+__device__ __forceinline__ bool sde_gt_half(double u) { return u > 0.5; }
+__device__ __forceinline__ bool sde_gt_half(SdeUnifX u) { return u.y >= 0x80000000u; }
+
+// f32 inverse normal of (y + 0.5) 2^-32 formed in the integer domain: q = min(u, 1 - u) is
+// (qi + 0.5) 2^-32 with qi = y or 2^32 - 1 - y. qi < 1233 (|z| > 5) takes the f64 path.
+__device__ __forceinline__ float sde_inv_normal_f32_x(sde_u32 y) {
+  const bool upper = y >= 0x80000000u;
+  const sde_u32 qi = upper ? ~y : y;
+  float x;
+  if (qi < 1233u) {
+    x = sde_inv_normal_tail(((double)qi + 0.5) * 0x1.0p-32);
+  } else {
+    const float q = __builtin_fmaf((float)qi, 0x1.0p-32f, 0x1.0p-33f);
+    const float t = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(q));
+    const float num = (0.010328f * t + 0.802853f) * t + 2.515517f;
+    const float den = ((0.001308f * t + 0.189269f) * t + 1.432788f) * t + 1.0f;
+    x = t - num * __builtin_amdgcn_rcpf(den);
+  }
+  return upper ? x : -x;
+}
+
 template <typename R> __device__ __forceinline__ R sde_psi(double p);
 template <> __device__ __forceinline__ double sde_psi<double>(double p) { return sde_inv_normal(p); }
 template <> __device__ __forceinline__ float sde_psi<float>(double p) { return sde_inv_normal_f32(p); }
+template <typename R> __device__ __forceinline__ R sde_psi(SdeUnifX u);
+template <> __device__ __forceinline__ double sde_psi<double>(SdeUnifX u) { return sde_inv_normal(sde_uval(u)); }
+template <> __device__ __forceinline__ float sde_psi<float>(SdeUnifX u) { return sde_inv_normal_f32_x(u.y); }
 
 // Sequential inverse Poisson CDF capped at k = 200 (f64, as the reference).
 __device__ __forceinline__ double sde_inv_poisson(double u, double lambda) {
@@ -152,7 +190,7 @@ template <typename R> __device__ __forceinline__ R sde_ne(R a, R b) { return sde
 template <typename R> __device__ __forceinline__ R sde_sq(R a) { return a * a; }
 
 __device__ __forceinline__ double sde_sqrt(double a) { return sqrt(a); }
-__device__ __forceinline__ float sde_sqrt(float a) { return __builtin_sqrtf(a); }
+__device__ __forceinline__ float sde_sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }  // v_sqrt_f32, 1 ulp
 __device__ __forceinline__ double sde_pow(double a, double b) { return pow(a, b); }
 __device__ __forceinline__ float sde_pow(float a, float b) { return powf(a, b); }
 __device__ __forceinline__ double sde_fmod(double a, double b) { return fmod(a, b); }
@@ -225,8 +263,10 @@ __device__ __forceinline__ void sde_block_reduce_store(double (&v)[NV], double* 
 //   prologue(X, C, K, t0)                       -- state of the cache and carried values at t0
 //   step(X, C, K, tc, tn, dt, sdt, u)           -- one step of one path; X becomes row n+1
 //   payoff(X, tT)                               -- terminal payoff (if HAS_PAYOFF)
-// Steps are processed in chunks of U = 8 / gcd(M, 8) so that the pseudo-random words of a
-// chunk are whole ChaCha blocks held in registers with compile-time word positions.
+// Pseudo-random steps are processed in chunks of U = 8 / gcd(M, 8) so that the words of a chunk
+// are whole ChaCha blocks held in registers with compile-time word positions. Sobol and
+// injected uniforms, and the time grid, are fetched one step ahead so that their memory latency
+// overlaps the current step's arithmetic.
 template <int M>
 struct SdeChunk {
   static constexpr int g8 = (M % 8 == 0) ? 8 : (M % 4 == 0) ? 4 : (M % 2 == 0) ? 2 : 1;
@@ -234,28 +274,58 @@ struct SdeChunk {
   static constexpr int BLOCKS = (M == 0) ? 1 : (U * M) / 8;
 };
 
-template <class Mdl, int RNG, int OUT, int PPL>
+template <int RNG> struct SdeUnifOf { typedef double type; };
+template <> struct SdeUnifOf<SDE_RNG_SOBOL_XOR> { typedef SdeUnifX type; };
+
+template <typename R>
+struct SdeGrid {
+  R tc, tn, dt, sdt;
+};
+
+// Time-grid values of step n. UG: the grid is uniform in the kernel's precision, so dt and
+// sqrt(dt) come from the argument block and only the time points the model reads are loaded.
+template <typename R, bool UG>
+__device__ __forceinline__ SdeGrid<R> sde_grid(const SimArgs& a, int n) {
+  SdeGrid<R> g;
+  if (sizeof(R) == 4) {
+    g.tc = (R)a.times_f[n]; g.tn = (R)a.times_f[n + 1];
+    g.dt = UG ? (R)a.dtf_u : (R)a.dts_f[n];
+    g.sdt = UG ? (R)a.sdtf_u : (R)a.sdts_f[n];
+  } else {
+    g.tc = (R)a.times[n]; g.tn = (R)a.times[n + 1];
+    g.dt = UG ? (R)a.dt_u : (R)a.dts[n];
+    g.sdt = UG ? (R)a.sdt_u : (R)a.sdts[n];
+  }
+  return g;
+}
+
+template <class Mdl, int RNG, int OUT, int PPL, int NSL, bool UG>
 __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
   typedef typename Mdl::real real;
+  typedef typename SdeUnifOf<RNG>::type unif;
   constexpr int P = Mdl::P;
   constexpr int M = Mdl::M;
   constexpr int MM = M > 0 ? M : 1;
   constexpr int NK = Mdl::NK;
   constexpr bool USE_CHACHA = (RNG == SDE_RNG_PSEUDO || RNG == SDE_RNG_SOBOL_SHIFT) && M > 0;
+  constexpr bool USE_SOBOL = (RNG == SDE_RNG_SOBOL_SHIFT || RNG == SDE_RNG_SOBOL_XOR) && M > 0;
+  constexpr bool USE_INJ = RNG == SDE_RNG_INJECTED && M > 0;
   constexpr int U = USE_CHACHA ? SdeChunk<M>::U : 1;
   constexpr int NB = SdeChunk<M>::BLOCKS;
-  constexpr bool F32 = sizeof(real) == 4;
   constexpr int KQ = P + (Mdl::HAS_PAYOFF ? 1 : 0);
+  constexpr bool FILT = (OUT & SDE_OUT_FILTRATION) != 0;
 
   const long long S = a.S;
   const int N = a.N;
   const long long first = (long long)blockIdx.x * ((long long)blockDim.x * PPL) + threadIdx.x;
+  const sde_u32 stride = 64u + 256u * (NSL - 1);
 
   real X[PPL][P], C[PPL][P], K[PPL][NK];
   sde_u32 key[PPL][8];
-  sde_u64 gray[PPL];
+  sde_u32 soff[PPL][NSL];  // byte offsets of this path's entries within a dimension's table row
   long long sid[PPL];
   bool live[PPL];
+  real* row[PPL];
 
 #pragma unroll
   for (int j = 0; j < PPL; ++j) {
@@ -266,18 +336,74 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
     for (int p = 0; p < P; ++p) { X[j][p] = (real)a.x0[p]; C[j][p] = X[j][p]; }
 #pragma unroll
     for (int k = 0; k < NK; ++k) K[j][k] = real(0);
-    Mdl::prologue(X[j], C[j], K[j], F32 ? (real)a.times_f[0] : (real)a.times[0]);
+    Mdl::prologue(X[j], C[j], K[j], sizeof(real) == 4 ? (real)a.times_f[0] : (real)a.times[0]);
     if (USE_CHACHA) sde_seed_from_u64(a.seed + (sde_u64)sid[j], key[j]);
-    if (RNG == SDE_RNG_SOBOL_SHIFT || RNG == SDE_RNG_SOBOL_XOR) {
+    if (USE_SOBOL) {
       const sde_u64 idx = (sde_u64)(a.sobol_first + sid[j]);
-      gray[j] = idx ^ (idx >> 1);
-    }
-    if ((OUT & SDE_OUT_FILTRATION) && live[j]) {
-      real* out = (real*)a.filtration;
+      sde_sobol_offsets<NSL>(idx ^ (idx >> 1), soff[j]);
 #pragma unroll
-      for (int p = 0; p < P; ++p) __builtin_nontemporal_store(X[j][p], out + (long long)p * S + sid[j]);
+      for (int sl = 0; sl < NSL; ++sl) soff[j][sl] *= 4u;
+    }
+    row[j] = FILT ? (real*)a.filtration + sid[j] : nullptr;
+    if (FILT && live[j]) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) __builtin_nontemporal_store(X[j][p], row[j] + (long long)p * S);
     }
   }
+  const long long row_stride = (long long)P * S;
+
+  // Sobol table entries are read through a buffer resource: the uniform row offset goes in the
+  // scalar offset and the path's fixed entry offsets in the 32-bit vector offset, so a gather
+  // costs no address arithmetic (and reads outside the table would return 0, not fault).
+  __amdgpu_buffer_rsrc_t tab_rsrc = __builtin_amdgcn_make_buffer_rsrc(
+      (void*)a.sobol_tab, (short)0, USE_SOBOL ? a.sobol_tab_words * 4 : 0, 0x00020000);
+
+  // Software pipeline for table-driven uniforms: the words of step n+1 are requested before
+  // step n is computed and combined into `nx` after it, so only one word per increment is
+  // carried from step to step.
+  sde_u32 sraw[PPL][MM][NSL];
+  sde_u32 nx[PPL][MM];
+  double ni[PPL][MM];
+  auto request = [&](int n) {
+#pragma unroll
+    for (int k = 0; k < MM; ++k) {
+      const int d = n * M + k;
+      if (USE_SOBOL) {
+        const int base = (int)((sde_u32)d * stride * 4u);
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+#pragma unroll
+          for (int sl = 0; sl < NSL; ++sl)
+            sraw[j][k][sl] = (sde_u32)__builtin_amdgcn_raw_buffer_load_b32(tab_rsrc, (int)soff[j][sl], base, 0);
+        }
+      }
+      if (USE_INJ) {
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) ni[j][k] = a.injected[(long long)d * S + sid[j]];
+      }
+    }
+  };
+  auto combine = [&](int n) {
+#pragma unroll
+    for (int k = 0; k < MM; ++k) {
+      if (USE_SOBOL) {
+        const sde_u32 mask = RNG == SDE_RNG_SOBOL_XOR ? a.xor_mask[n * M + k] : 0u;
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) {
+          sde_u32 x = mask;
+#pragma unroll
+          for (int sl = 0; sl < NSL; ++sl) x ^= sraw[j][k][sl];
+          nx[j][k] = x;
+        }
+      }
+    }
+  };
+  if (N > 0) {
+    request(0);
+    combine(0);
+  }
+  SdeGrid<real> g{};
+  if (N > 0) g = sde_grid<real, UG>(a, 0);
 
   for (int n0 = 0; n0 < N; n0 += U) {
     sde_u32 W[PPL][USE_CHACHA ? NB * 16 : 1];
@@ -297,49 +423,49 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
     for (int i = 0; i < U; ++i) {
       const int n = n0 + i;
       if (U == 1 || n < N) {
-        real tc, tn, dt, sdt;
-        if (F32) {
-          tc = (real)a.times_f[n]; tn = (real)a.times_f[n + 1]; dt = (real)a.dts_f[n]; sdt = (real)a.sdts_f[n];
-        } else {
-          tc = (real)a.times[n]; tn = (real)a.times[n + 1]; dt = (real)a.dts[n]; sdt = (real)a.sdts[n];
-        }
+        unif u[PPL][MM];
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
-          double u[MM];
 #pragma unroll
           for (int k = 0; k < MM; ++k) {
-            if (M == 0) { u[k] = 0.5; continue; }
-            const int d = n * M + k;
-            if (RNG == SDE_RNG_PSEUDO) {
+            double v = 0.5;
+            if (RNG == SDE_RNG_PSEUDO && M > 0) {
               const int w = 2 * (i * M + k);
-              u[k] = sde_u01_from_words(W[j][USE_CHACHA ? w : 0], W[j][USE_CHACHA ? w + 1 : 0]);
-            } else if (RNG == SDE_RNG_SOBOL_SHIFT) {
+              v = sde_u01_from_words(W[j][USE_CHACHA ? w : 0], W[j][USE_CHACHA ? w + 1 : 0]);
+            } else if (RNG == SDE_RNG_SOBOL_SHIFT && M > 0) {
               const int w = 2 * (i * M + k);
               const double sh = sde_u01_from_words(W[j][USE_CHACHA ? w : 0], W[j][USE_CHACHA ? w + 1 : 0]);
-              const double x = (double)sde_sobol_coord(a.sobol_tab, a.nslices, d, gray[j]) * 0x1.0p-32;
-              const double y = x + sh;
-              u[k] = y - trunc(y);
-            } else if (RNG == SDE_RNG_SOBOL_XOR) {
-              const sde_u32 y = sde_sobol_coord(a.sobol_tab, a.nslices, d, gray[j]) ^ a.xor_mask[d];
-              u[k] = ((double)y + 0.5) * 0x1.0p-32;
-            } else if (RNG == SDE_RNG_INJECTED) {
-              u[k] = a.injected[(long long)d * S + sid[j]];
+              const double y = (double)nx[j][k] * 0x1.0p-32 + sh;
+              v = y - trunc(y);
+            } else if (USE_INJ) {
+              v = ni[j][k];
+            }
+            if constexpr (RNG == SDE_RNG_SOBOL_XOR) {
+              u[j][k].y = nx[j][k];
             } else {
-              u[k] = 0.5;
+              u[j][k] = v;
             }
           }
-          Mdl::step(X[j], C[j], K[j], tc, tn, dt, sdt, u);
         }
-        if (OUT & SDE_OUT_FILTRATION) {
-          real* out = (real*)a.filtration + (long long)(n + 1) * P * S;
+        const SdeGrid<real> gc = g;
+        const bool more = n + 1 < N;
+        if (more) {
+          g = sde_grid<real, UG>(a, n + 1);
+          request(n + 1);
+        }
+#pragma unroll
+        for (int j = 0; j < PPL; ++j) Mdl::step(X[j], C[j], K[j], gc.tc, gc.tn, gc.dt, gc.sdt, u[j]);
+        if (FILT) {
 #pragma unroll
           for (int j = 0; j < PPL; ++j) {
+            row[j] += row_stride;
             if (live[j]) {
 #pragma unroll
-              for (int p = 0; p < P; ++p) __builtin_nontemporal_store(X[j][p], out + (long long)p * S + sid[j]);
+              for (int p = 0; p < P; ++p) __builtin_nontemporal_store(X[j][p], row[j] + (long long)p * S);
             }
           }
         }
+        if (more) combine(n + 1);
       }
     }
   }
@@ -348,7 +474,7 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
     double acc[2 * KQ];
 #pragma unroll
     for (int q = 0; q < 2 * KQ; ++q) acc[q] = 0.0;
-    const real tT = F32 ? (real)a.times_f[N] : (real)a.times[N];
+    const real tT = sizeof(real) == 4 ? (real)a.times_f[N] : (real)a.times[N];
 #pragma unroll
     for (int j = 0; j < PPL; ++j) {
       if (live[j]) {

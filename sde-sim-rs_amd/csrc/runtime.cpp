@@ -115,7 +115,7 @@ std::vector<char> rtc_compile(const std::string& src) {
   return code;
 }
 
-CompiledKernel compile_kernel(const std::string& src) {
+CompiledKernel compile_kernel(const std::string& src, const std::string& name) {
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
   std::lock_guard<std::mutex> lock(g_cache_mu);
@@ -125,7 +125,7 @@ CompiledKernel compile_kernel(const std::string& src) {
   std::vector<char> code = rtc_compile(src);
   CompiledKernel k;
   hip_check(hipModuleLoadData(&k.module, code.data()), "hipModuleLoadData");
-  hip_check(hipModuleGetFunction(&k.function, k.module, "sdesim_kernel"), "hipModuleGetFunction");
+  hip_check(hipModuleGetFunction(&k.function, k.module, name.c_str()), "hipModuleGetFunction");
   g_kernels[key] = k;
   return k;
 }
@@ -155,6 +155,8 @@ void direction_numbers(int d, unsigned int V[32]) {
   for (int k = 1; k <= 32; ++k) V[k - 1] = (unsigned int)(m[k] << (32 - k));
 }
 
+void build_sobol_tables(int want, int nslices, std::vector<unsigned int>& host);
+
 struct SobolTables {
   int dims = 0, nslices = 0;
   unsigned int* dev = nullptr;
@@ -170,8 +172,23 @@ SobolTables sobol_tables(int dims, int nslices) {
   SobolTables& t = g_sobol[std::make_pair(dev, nslices)];
   if (t.dev && t.dims >= dims) return t;
   int want = std::max(dims, t.dims);
+  std::vector<unsigned int> host;
+  build_sobol_tables(want, nslices, host);
+  // The previous tables may still be read by queued kernels; they are kept until exit.
+  unsigned int* d_tab = nullptr;
+  hip_check(hipMalloc(&d_tab, host.size() * sizeof(unsigned int)), "hipMalloc(sobol tables)");
+  hip_check(hipMemcpy(d_tab, host.data(), host.size() * sizeof(unsigned int), hipMemcpyHostToDevice),
+            "hipMemcpy(sobol tables)");
+  t.dev = d_tab;
+  t.dims = want;
+  t.nslices = nslices;
+  return t;
+}
+
+// Sliced XOR tables [dims][64 + 256 (nslices - 1)] of the direction numbers.
+void build_sobol_tables(int want, int nslices, std::vector<unsigned int>& host) {
   const int stride = 64 + 256 * (nslices - 1);
-  std::vector<unsigned int> host((size_t)want * stride);
+  host.assign((size_t)want * stride, 0u);
   for (int d = 0; d < want; ++d) {
     unsigned int V[32];
     direction_numbers(d, V);
@@ -193,15 +210,6 @@ SobolTables sobol_tables(int dims, int nslices) {
       }
     }
   }
-  // The previous tables may still be read by queued kernels; they are kept until exit.
-  unsigned int* d_tab = nullptr;
-  hip_check(hipMalloc(&d_tab, host.size() * sizeof(unsigned int)), "hipMalloc(sobol tables)");
-  hip_check(hipMemcpy(d_tab, host.data(), host.size() * sizeof(unsigned int), hipMemcpyHostToDevice),
-            "hipMemcpy(sobol tables)");
-  t.dev = d_tab;
-  t.dims = want;
-  t.nslices = nslices;
-  return t;
 }
 
 // Words of ChaCha8Rng::seed_from_u64(seed): the XOR masks of the digital shift.
@@ -357,6 +365,18 @@ int sdesim_plan_create(const sdesim_universe* uh, const double* times, int32_t T
     else if (opt.rng == SDESIM_INJECTED) rng_kind = SDE_RNG_INJECTED;
     else rng_kind = SDE_RNG_PSEUDO;  // the reference treats any other string as "pseudo"
 
+    int nslices = 1;
+    long long sobol_first = 0;
+    if (rng_kind == SDE_RNG_SOBOL_SHIFT || rng_kind == SDE_RNG_SOBOL_XOR) {
+      sobol_first = opt.sobol_first >= 0 ? opt.sobol_first : (rng_kind == SDE_RNG_SOBOL_XOR ? 0 : 5);
+      const unsigned long long last = (unsigned long long)(sobol_first + S - 1);
+      const unsigned long long g = last | (last >> 1);
+      int bits = 0;
+      while (bits < 64 && (g >> bits) != 0ull) ++bits;
+      if (bits > 32) fail(SDESIM_EPANIC, "Sobol sequence exhausted");
+      while (6 + 8 * (nslices - 1) < bits) ++nslices;
+    }
+
     sdesim::CodegenOptions co;
     co.scheme = opt.scheme;
     co.compat = opt.compat;
@@ -365,10 +385,9 @@ int sdesim_plan_create(const sdesim_universe* uh, const double* times, int32_t T
     co.out = opt.outputs & (SDESIM_OUT_FILTRATION | SDESIM_OUT_TERMINAL);
     co.ppl = plan->ppl;
     co.block = plan->block;
+    co.nsl = nslices;
     co.steps = N > 0;
     co.payoff = has_payoff ? payoff : "";
-    plan->source = sdesim::generate_kernel_source(U, co);
-    plan->kernel = compile_kernel(plan->source);
 
     // Time grid in the reference's arithmetic: dt = t[n+1] - t[n], sqrt(dt) (increment.rs:38-41,75-79).
     std::vector<double> tv(times, times + T), dts(N), sdts(N);
@@ -376,6 +395,15 @@ int sdesim_plan_create(const sdesim_universe* uh, const double* times, int32_t T
     for (int n = 0; n < N; ++n) { dts[n] = tv[n + 1] - tv[n]; sdts[n] = std::sqrt(dts[n]); }
     for (int i = 0; i < T; ++i) tvf[i] = (float)tv[i];
     for (int n = 0; n < N; ++n) { dtsf[n] = (float)dts[n]; sdtsf[n] = (float)sdts[n]; }
+    bool ug = N > 0;
+    for (int n = 1; n < N && ug; ++n) {
+      if (co.f32) ug = std::memcmp(&dtsf[n], &dtsf[0], 4) == 0 && std::memcmp(&sdtsf[n], &sdtsf[0], 4) == 0;
+      else ug = std::memcmp(&dts[n], &dts[0], 8) == 0 && std::memcmp(&sdts[n], &sdts[0], 8) == 0;
+    }
+    co.uniform_grid = ug;
+    co.name = sdesim::kernel_name(co);
+    plan->source = sdesim::generate_kernel_source(U, co);
+    plan->kernel = compile_kernel(plan->source, co.name);
     std::vector<double> x0v(x0 ? x0 : nullptr, x0 ? x0 + P : nullptr);
     if (!x0) x0v.assign(P, 0.0);
     if (x0v.empty()) x0v.push_back(0.0);
@@ -388,24 +416,20 @@ int sdesim_plan_create(const sdesim_universe* uh, const double* times, int32_t T
     a.dts_f = plan->own(upload(dtsf));
     a.sdts_f = plan->own(upload(sdtsf));
     a.x0 = plan->own(upload(x0v));
+    if (N > 0) {
+      a.dt_u = dts[0]; a.sdt_u = sdts[0];
+      a.dtf_u = dtsf[0]; a.sdtf_u = sdtsf[0];
+    }
     a.S = S;
     a.seed = opt.seed;
     a.N = N;
     a.T = T;
-    a.nslices = 1;
+    a.nslices = nslices;
+    a.sobol_first = sobol_first;
     if (rng_kind == SDE_RNG_SOBOL_SHIFT || rng_kind == SDE_RNG_SOBOL_XOR) {
-      long long first = opt.sobol_first >= 0 ? opt.sobol_first : (rng_kind == SDE_RNG_SOBOL_XOR ? 0 : 5);
-      a.sobol_first = first;
-      unsigned long long last = (unsigned long long)(first + S - 1);
-      unsigned long long g = last | (last >> 1);
-      int bits = 0;
-      while (bits < 64 && (g >> bits) != 0ull) ++bits;
-      if (bits > 32) fail(SDESIM_EPANIC, "Sobol sequence exhausted");
-      int nslices = 1;
-      while (6 + 8 * (nslices - 1) < bits) ++nslices;
       SobolTables st = sobol_tables(std::max(1, N * M), nslices);
       a.sobol_tab = st.dev;
-      a.nslices = nslices;
+      a.sobol_tab_words = st.dims * (64 + 256 * (nslices - 1));
       if (rng_kind == SDE_RNG_SOBOL_XOR) {
         std::vector<unsigned int> masks;
         chacha8_words(opt.scramble_seed, (size_t)std::max(1, N * M), masks);
@@ -543,6 +567,9 @@ int sdesim_generate_source(const sdesim_universe* u, int32_t n_times, const sdes
     co.out = opt.outputs & (SDESIM_OUT_FILTRATION | SDESIM_OUT_TERMINAL);
     co.ppl = opt.paths_per_lane > 0 ? opt.paths_per_lane : 1;
     co.block = opt.block_size > 0 ? opt.block_size : 256;
+    co.nsl = 3;
+    co.uniform_grid = true;
+    co.name = sdesim::kernel_name(co);
     co.steps = n_times > 1;
     co.payoff = payoff ? payoff : "";
     std::string src = sdesim::generate_kernel_source(u->u, co);
@@ -555,6 +582,47 @@ int sdesim_compile_source(const char* source, int64_t* code_bytes) {
   return guarded([&] {
     std::vector<char> code = rtc_compile(source ? source : "");
     if (code_bytes) *code_bytes = (int64_t)code.size();
+  });
+}
+
+int sdesim_eval_expression(const char* expr, const char* const* names, const double* values, int32_t n,
+                           double* out) {
+  return guarded([&] {
+    sdesim::ExprPtr e = sdesim::parse_expression(expr ? expr : "");
+    std::map<std::string, double> ns;
+    for (int32_t i = 0; i < n; ++i) ns[names[i]] = values[i];
+    try {
+      *out = sdesim::eval_host(*e, ns);
+    } catch (const std::runtime_error& err) {
+      fail(SDESIM_EPANIC, err.what());
+    }
+  });
+}
+
+int sdesim_sobol_points(int32_t dims, uint64_t first, int64_t n, uint32_t* out) {
+  return guarded([&] {
+    if (dims <= 0 || dims > sdesim_jk_dims) fail(SDESIM_EVALUE, "dims out of range");
+    if (n < 0) fail(SDESIM_EVALUE, "negative count");
+    const unsigned long long last = first + (unsigned long long)(n > 0 ? n - 1 : 0);
+    const unsigned long long gl = last | (last >> 1);
+    int bits = 0;
+    while (bits < 64 && (gl >> bits) != 0ull) ++bits;
+    if (bits > 32) fail(SDESIM_EPANIC, "Sobol sequence exhausted");
+    int nsl = 1;
+    while (6 + 8 * (nsl - 1) < bits) ++nsl;
+    std::vector<unsigned int> tab;
+    build_sobol_tables(dims, nsl, tab);
+    const int stride = 64 + 256 * (nsl - 1);
+    for (int64_t i = 0; i < n; ++i) {
+      const unsigned long long idx = first + (unsigned long long)i;
+      const unsigned long long g = idx ^ (idx >> 1);
+      for (int d = 0; d < dims; ++d) {
+        const unsigned int* row = &tab[(size_t)d * stride];
+        unsigned int x = row[g & 63u];
+        for (int k = 1; k < nsl; ++k) x ^= row[64 + 256 * (k - 1) + ((g >> (6 + 8 * (k - 1))) & 255u)];
+        out[(size_t)i * dims + d] = x;
+      }
+    }
   });
 }
 

@@ -63,6 +63,11 @@ SIGNATURES = {
                                               ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64,
                                               ctypes.POINTER(ctypes.c_int64)]),
     "sdesim_compile_source": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
+    "sdesim_eval_expression": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_char_p),
+                                              ctypes.POINTER(ctypes.c_double), ctypes.c_int32,
+                                              ctypes.POINTER(ctypes.c_double)]),
+    "sdesim_sobol_points": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
+                                           ctypes.POINTER(ctypes.c_uint32)]),
     "sdesim_last_error": (ctypes.c_char_p, []),
     "sdesim_version": (ctypes.c_int32, []),
 }

@@ -1,0 +1,270 @@
+"""Host-side product tests (CPU): the C ABI, the DSL parser, expression semantics, code
+generation (hipRTC compiles for gfx950 without a device), Sobol tables and error behaviour."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import GBM, HESTON, JUMP, JUMP_X0, ROOT
+
+import sde_sim_rs
+from sde_sim_rs import PanicException
+from sde_sim_rs import _native as nat
+
+
+def test_abi_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "sdesim.h")).read()
+    decls = set(re.findall(r"\b(sdesim_[a-z_]+)\s*\(", hdr))
+    assert len(decls) >= 20
+    lib = ctypes.CDLL(nat.LIB_PATH)
+    for name in sorted(decls):
+        assert hasattr(lib, name), name
+        assert name in nat.SIGNATURES, name
+    assert set(nat.SIGNATURES) == decls
+
+
+# ---------------------------------------------------------------------------------------------
+# DSL (src/proc/util.rs): product parser against the oracle's
+# ---------------------------------------------------------------------------------------------
+UNIVERSES = [
+    GBM, HESTON, JUMP,
+    ["dX1 = ( sin(t) ) * dt + (0.01 * X1) * dW1 + (0.001 * X1) * dN1(0.5 * cos(t))", "X2 = max(X1 - 100.0, 0.0)"],
+    ["dX = (1.0) * dt - (2.0) * dW2 + (3) * dW1", "dY = (X) * dW1"],
+    ["dX = (1.0) * dt + (2.0) dW1"],                 # stops quietly at ')' not followed by '*'
+    ["dX = 0.05 * X * dt"],                          # no parenthesised coefficient: no terms
+    ["dA = (1) * dN1(A)", "dB = (1) * dN1( A )", "dC = (1) * dN1(A)"],  # dN keys include lambda text
+    ["Y = 3", "X = Y * 2"],
+    ["dX = ((1 + 2) * (3)) * dt"],
+]
+
+
+@pytest.mark.parametrize("eqs", UNIVERSES)
+def test_parser_matches_oracle(oracle, eqs):
+    u = sde_sim_rs.Universe(eqs)
+    o = oracle.parse(eqs)
+    assert u.num_processes == o["P"]
+    assert u.num_increments == o["m"]
+    assert u.names == o["names"]
+    assert u.is_algebraic == o["algebraic"]
+
+
+def test_parser_registry_order_and_names():
+    u = sde_sim_rs.Universe(["dX = (1.0) * dt - (2.0) * dW2 + (3) * dW1", "dY = (X) * dW1"])
+    assert u.names == ["X", "Y"] and u.num_increments == 2
+    u = sde_sim_rs.Universe(["dA = (1) * dN1(A)", "dB = (1) * dN1( A )", "dC = (1) * dN1(A)"])
+    assert u.num_increments == 2
+    u = sde_sim_rs.Universe(["dX = (1) * dt", "dX = (2) * dt"])
+    assert u.index("X") == 1  # the registry keeps the last index of a repeated name
+
+
+BAD = [
+    ("dX = (0.1 * X * dt", "Unbalanced parentheses in coefficient"),
+    ("dX = (1) * dZ", "Unknown incrementor type: dZ"),
+    ("X", "Missing '='"),
+    ("a = b = c", "Missing '='"),
+    ("dX = (1) * dN1(X", "Unbalanced parentheses in dN intensity"),
+    ("dX = (1) * dN1", "dN missing opening bracket"),
+    ("dX = (2 +) * dt", "Math error in coefficient"),
+    ("dX=(1)*dt+(2)*dW1", "Unknown incrementor type: dt+(2)*dW1"),
+    ("X = (1 + ", "Parse Error"),
+    ("dX = (1) * dN1(1 +)", "Math error in jump lambda '1 +'"),
+]
+
+
+@pytest.mark.parametrize("eq,msg", BAD)
+def test_parse_errors(oracle, eq, msg):
+    with pytest.raises(ValueError) as e:
+        sde_sim_rs.Universe([eq])
+    assert str(e.value).startswith("Failed to parse equations: ")
+    assert msg in str(e.value)
+    with pytest.raises(ValueError) as e2:
+        oracle.parse([eq])
+    assert msg in str(e2.value)
+
+
+# ---------------------------------------------------------------------------------------------
+# Expression semantics (fasteval restatement): product host evaluator vs oracle evaluator
+# ---------------------------------------------------------------------------------------------
+def product_eval(expr, ns):
+    names = list(ns)
+    arr = (ctypes.c_char_p * max(1, len(names)))(*[n.encode() for n in names])
+    vals = np.ascontiguousarray([float(ns[n]) for n in names] or [0.0])
+    out = ctypes.c_double()
+    nat.check(nat.lib.sdesim_eval_expression(expr.encode(), arr, nat.dptr(vals), len(names), ctypes.byref(out)),
+              value_prefix="")
+    return out.value
+
+
+EXPRS = [
+    ("2 - 3 + 4", 3.0), ("2 + 3 - 4", 1.0), ("2 * 3 / 4", 1.5), ("2 / 4 * 3", 1.5), ("2 ^ 3 ^ 2", 512.0),
+    ("-2 ^ 2", 4.0), ("10 % 4 * 2", 4.0), ("1 + 2 * 3", 7.0), ("(1 + 2) * 3", 9.0), ("1 < 2 == 1", 1.0),
+    ("2k", 2000.0), ("3m", 0.003), ("log(100)", 2.0), ("log(2, 8)", 3.0), ("round(0.5, 1.26)", 1.5),
+    ("min(3, 1, 2)", 1.0), ("max(3, x, 2)", 5.0), ("0 && 5", 0.0), ("2 && 5", 5.0), ("0 || 7", 7.0),
+    ("3 || 7", 3.0), ("!0", 1.0), ("!3", 0.0), ("int(-2.7)", -2.0), ("sign(-0.5)", -1.0), ("abs(-3)", 3.0),
+    ("e()", np.e), ("pi()", np.pi), ("x(1, 2)", 5.0), ("1.5e3", 1500.0), (".5 + 1.", 1.5), ("+-+1", -1.0),
+    ("-7 % 3", -1.0), ("2 ^ -1", 0.5), ("floor(-1.5) + ceil(1.2)", 0.0), ("t * 2", 0.5), ("3 == 3 + 1e-16", 1.0),
+]
+
+
+@pytest.mark.parametrize("expr,val", EXPRS)
+def test_expression_semantics(oracle, expr, val):
+    ns = {"x": 5.0, "t": 0.25}
+    p = product_eval(expr, ns)
+    o = oracle.eval_expr(expr, ns)
+    assert p == o or (np.isnan(p) and np.isnan(o))
+    assert p == pytest.approx(val, rel=1e-12, abs=1e-15)
+
+
+def test_expression_undefined_name_panics(oracle):
+    with pytest.raises(PanicException):
+        product_eval("y + 1", {"x": 1.0})
+    with pytest.raises(RuntimeError):
+        oracle.eval_expr("y + 1", {"x": 1.0})
+
+
+def test_expression_random_agreement(oracle):
+    """Random expressions over the operator set: the two independent parsers/evaluators agree
+    bit for bit; on the subset without '-' and '/' (where fasteval's precedence coincides with
+    Python's) they also agree with Python's evaluation."""
+    rng = np.random.default_rng(11)
+    ops = ["+", "-", "*", "/", "%", "^", "<", ">", "<=", ">=", "==", "!=", "&&", "||"]
+
+    def gen(depth, allowed):
+        if depth == 0 or rng.random() < 0.3:
+            c = rng.integers(0, 4)
+            if c == 0:
+                return "a"
+            if c == 1:
+                return "b"
+            return "%.3g" % rng.uniform(0.1, 3)
+        if rng.random() < 0.15:
+            return "(" + gen(depth - 1, allowed) + ")"
+        if rng.random() < 0.1:
+            return rng.choice(["min", "max"]) + "(" + gen(depth - 1, allowed) + ", " + gen(depth - 1, allowed) + ")"
+        return gen(depth - 1, allowed) + " " + rng.choice(allowed) + " " + gen(depth - 1, allowed)
+
+    ns = {"a": 1.7, "b": 0.6}
+    for _ in range(400):
+        e = gen(4, ops)
+        try:
+            o = oracle.eval_expr(e, ns)
+        except (ValueError, RuntimeError):
+            continue
+        p = product_eval(e, ns)
+        assert (p == o) or (np.isnan(p) and np.isnan(o)), e
+    for _ in range(300):
+        e = gen(4, ["+", "*"])
+        p = product_eval(e, ns)
+        py = eval(e, {"min": min, "max": max}, dict(ns))
+        assert p == pytest.approx(py, rel=1e-12), e
+
+
+# ---------------------------------------------------------------------------------------------
+# Sobol tables the kernels read, against scipy's generator (golden fixtures)
+# ---------------------------------------------------------------------------------------------
+def product_sobol(dims, n, first=0):
+    out = np.zeros((n, dims), dtype=np.uint32)
+    nat.check(nat.lib.sdesim_sobol_points(dims, first, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32))))
+    return out
+
+
+def test_device_sobol_tables_match_scipy():
+    g = np.load(os.path.join(ROOT, "tests", "golden", "sobol_scipy.npz"))
+    assert np.array_equal(product_sobol(2048, 300)[:, g["dims2048_cols"]], g["dims2048_first300"])
+    assert np.array_equal(product_sobol(2048, 64, 1000)[:, g["ff1000_cols"]], g["dims2048_from1000"])
+    assert np.array_equal(product_sobol(21201, 64)[:, g["dims21201_cols"]], g["dims21201_first64"])
+
+
+def test_device_sobol_tables_far_indices(oracle):
+    # indices that need 3, 4 and 5 table slices
+    for first in (1 << 20, (1 << 27) + 12345, (1 << 31) + 777):
+        assert np.array_equal(product_sobol(16, 32, first), oracle.sobol_points(16, 32, first=first))
+
+
+# ---------------------------------------------------------------------------------------------
+# Code generation: every variant compiles for gfx950 (hipRTC, no device needed)
+# ---------------------------------------------------------------------------------------------
+def generate(eqs, n_times, payoff=None, **opts):
+    u = sde_sim_rs.Universe(eqs)
+    o = nat.default_options()
+    for k, v in opts.items():
+        setattr(o, k, v)
+    need = ctypes.c_int64()
+    pb = payoff.encode() if payoff else None
+    nat.check(nat.lib.sdesim_generate_source(u._h, n_times, ctypes.byref(o), pb, None, 0, ctypes.byref(need)))
+    buf = ctypes.create_string_buffer(need.value)
+    nat.check(nat.lib.sdesim_generate_source(u._h, n_times, ctypes.byref(o), pb, buf, need.value, ctypes.byref(need)))
+    return buf.value.decode()
+
+
+@pytest.mark.parametrize("eqs", [GBM, HESTON, JUMP], ids=["gbm", "heston", "jump"])
+@pytest.mark.parametrize("scheme", [0, 1])
+@pytest.mark.parametrize("precision", [0, 1])
+@pytest.mark.parametrize("rng", [(0, 0), (1, 1)], ids=["pseudo", "sobolxor"])
+def test_generated_kernels_compile(eqs, scheme, precision, rng):
+    src = generate(eqs, 33, payoff=None, scheme=scheme, precision=precision, rng=rng[0], scramble=rng[1],
+                   outputs=3 if precision else 1)
+    size = ctypes.c_int64()
+    rc = nat.lib.sdesim_compile_source(src.encode(), ctypes.byref(size))
+    assert rc == 0, nat.last_error()[:2000]
+    assert size.value > 1000
+
+
+def test_codegen_reference_rk_carries_probe_coefficients():
+    # RK with reference compat: the stage-1 coefficients of step n are the stage-2 values of step
+    # n-1 (same expressions on the same probe cache at the same time), carried in registers.
+    src = generate(HESTON, 33, scheme=1, precision=1, rng=1, scramble=1)
+    body = src[src.index("void step"):]
+    assert "K[" in body and body.count("sde_sqrt(") == 3
+    src_tb = generate(HESTON, 33, scheme=1, precision=1, rng=1, scramble=1, compat=1)
+    assert src_tb[src_tb.index("void step"):].count("sde_sqrt(") == 6
+
+
+def test_codegen_payoff_and_names():
+    src = generate(HESTON, 33, payoff="max(S - 100, 0)", scheme=1, precision=1, rng=1, scramble=1, outputs=2)
+    assert "sdesim_rk_f32_sobolxor_term" in src and "X[0]" in src[src.index("payoff("):]
+
+
+# ---------------------------------------------------------------------------------------------
+# Error behaviour of the Python API (src/py_binding.rs:19-47), raised before any device work
+# ---------------------------------------------------------------------------------------------
+def test_scenarios_must_be_positive():
+    with pytest.raises(ValueError, match="scenarios must be a positive integer"):
+        sde_sim_rs.simulate(GBM, [0.0, 1.0], 0, {"X": 1.0}, "pseudo", "euler")
+    with pytest.raises(TypeError):
+        sde_sim_rs.simulate(GBM, [0.0, 1.0], 1.5, {"X": 1.0}, "pseudo", "euler")
+    with pytest.raises(OverflowError):
+        sde_sim_rs.simulate(GBM, [0.0, 1.0], 2 ** 31, {"X": 1.0}, "pseudo", "euler")
+    with pytest.raises(TypeError):
+        sde_sim_rs.simulate("dX = (1) * dt", [0.0, 1.0], 1, {"X": 1.0}, "pseudo", "euler")
+
+
+def test_parse_error_through_simulate():
+    with pytest.raises(ValueError, match="^Failed to parse equations: Unknown incrementor type: dQ"):
+        sde_sim_rs.simulate(["dX = (1) * dQ"], [0.0, 1.0], 1, {}, "pseudo", "euler")
+
+
+def test_panics_where_the_reference_panics():
+    with pytest.raises(PanicException, match="not implemented"):
+        sde_sim_rs.simulate(GBM, [0.0, 1.0], 1, {"X": 1.0}, "pseudo", "milstein")
+    with pytest.raises(PanicException, match="Undefined|undefined"):
+        sde_sim_rs.simulate(["dX = (q) * dt"], [0.0, 1.0], 1, {}, "pseudo", "euler")
+    with pytest.raises(PanicException, match="RNG Index 0 out of bounds"):
+        sde_sim_rs.simulate(["dX = (1) * dt"], [0.0, 1.0], 1, {}, "pseudo", "runge-kutta")
+    with pytest.raises(PanicException):
+        sde_sim_rs.simulate(GBM, [], 1, {"X": 1.0}, "pseudo", "euler")
+    with pytest.raises(PanicException, match="Sobol dimension"):
+        sde_sim_rs.simulate(HESTON, np.linspace(0, 1, 12000), 1, {}, "sobol", "euler")
+
+
+def test_duplicate_time_points_rejected():
+    with pytest.raises(ValueError, match="duplicate"):
+        sde_sim_rs.simulate(GBM, [0.0, 0.5, 0.5, 1.0], 1, {"X": 1.0}, "pseudo", "euler")
+
+
+def test_empty_universe_returns_empty_table():
+    t = sde_sim_rs.simulate([], [0.0, 1.0], 3, {}, "pseudo", "euler")
+    tab = t if hasattr(t, "column") else t.to_arrow()
+    assert tab.num_rows == 0 and tab.column_names == ["scenario", "time", "process_name", "value"]
