@@ -80,9 +80,9 @@ def main():
     ap.add_argument("--nsteps", type=int, default=1024)
     ap.add_argument("--ppl", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--cpu-paths", type=int, default=1 << 15)
+    ap.add_argument("--cpu-paths", type=int, default=1 << 17)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", default="", help="comma list of legs to run: f32,f64,term (profiling)")
+    ap.add_argument("--only", default="", help="comma list of legs to run: f32,f64,term,batch (profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: paths are independent and the multi-GPU form "
@@ -96,7 +96,7 @@ def main():
 
     S, N = args.paths, args.nsteps
     times = np.linspace(0.0, 1.0, N + 1)
-    legs = set(args.only.split(",")) if args.only else {"f32", "f64", "term"}
+    legs = set(args.only.split(",")) if args.only else {"f32", "f64", "term", "batch"}
     common = dict(seed=0, scramble="xor", scramble_seed=7, paths_per_lane=args.ppl, block_size=args.block)
     uni = sde_sim_rs.Universe(HESTON)
     res = {}
@@ -121,6 +121,31 @@ def main():
             term[prec] = {"ms": k, "path_steps_per_s": S * N / (k * 1e-3), "payoff_mean": m["payoff"][0],
                           "payoff_var": m["payoff"][1], "S_mean": m["S"][0]}
         torch.cuda.synchronize()
+
+    batch = None
+    if "batch" in legs:
+        from sde_sim_rs import models
+        insts = models.batched_instances(n=4096, seed=11)
+        bt = sde_sim_rs.Batch(insts, precision="f32")
+        ppl, blk, tune_ms = bt.autotune()
+        bt.capture()
+        stream = torch.cuda.current_stream()
+        for _ in range(args.warmup):
+            bt.replay(stream)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        reps = max(1, args.steps // 4)
+        e0.record(stream)
+        for _ in range(reps):
+            bt.replay(stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / reps
+        info = bt.info()
+        batch = {"workload": "4096 batched GBM/OU/CIR instances (seed 11), Euler, f32, terminal moments",
+                 "path_steps": bt.path_steps, "ms_per_launch": ms, "path_steps_per_s": bt.path_steps / (ms * 1e-3),
+                 "graph": True, "paths_per_lane": ppl, "block_size": blk, "grid": info["grid"],
+                 "work_items": info["work_items"]}
 
     path_steps = S * N
     line = {"metric": "path-steps/s on 1 MI355X, FP32 vs FP64 plain at equal error; HBM GB/s %peak",
@@ -151,6 +176,8 @@ def main():
         term["fp32_payoff_rel_err_vs_fp64"] = e
         term["fp32_over_fp64"] = term["f64"]["ms"] / term["f32"]["ms"]
         line["terminal_only"] = term
+    if batch:
+        line["batched"] = batch
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         v, dt = cpu_baseline(args.cpu_paths, N, threads)

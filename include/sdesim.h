@@ -137,6 +137,49 @@ int sdesim_eval_expression(const char* expr, const char* const* names, const dou
  * before its f64 conversion x * 2^-32). */
 int sdesim_sobol_points(int32_t dims, uint64_t first, int64_t n, uint32_t* out);
 
+/* ---------------------------------------------------------------------------------------
+ * Batches: many independent 1-D instances of uneven size in one persistent launch.
+ * No reference counterpart (the reference runs one universe per simulate call,
+ * src/sim/mod.rs:20-92); each instance is what simulate would compute for its built-in
+ * model's DSL form on the uniform grid t_n = n * horizon / steps with seed `seed`:
+ *   kind 0 GBM  dX = (p0 * X) * dt + (p1 * X) * dW1
+ *   kind 1 OU   dX = (p0 * (p1 - X)) * dt + (p2) * dW1
+ *   kind 2 CIR  dX = (p0 * (p1 - max(X, 0))) * dt + (p2 * max(X, 0)^0.5) * dW1
+ * Euler-Maruyama, pseudo-random (ChaCha8, path s seeded seed + s), terminal mean and variance.
+ * --------------------------------------------------------------------------------------- */
+typedef struct sdesim_batch sdesim_batch;
+
+typedef struct sdesim_instance {
+  int32_t kind;    /* 0 GBM, 1 OU, 2 CIR */
+  int32_t steps;   /* N >= 1 */
+  int64_t paths;   /* S >= 1 */
+  double x0;
+  double horizon;  /* T > 0 */
+  double p0, p1, p2;
+  uint64_t seed;
+} sdesim_instance;
+
+/* keep_terminal != 0 keeps every path's terminal value (instance-major) for sdesim_batch_terminal. */
+int sdesim_batch_create(const sdesim_instance* instances, int32_t n, int32_t precision, int32_t keep_terminal,
+                        sdesim_batch** out);
+void sdesim_batch_free(sdesim_batch* b);
+/* paths_per_lane in {1, 2, 4}, block_size in {64, 128, 256}; drops a captured graph. */
+int sdesim_batch_set_config(sdesim_batch* b, int32_t paths_per_lane, int32_t block_size);
+/* Time every configuration on this batch (or reuse a remembered choice for this device and
+ * workload shape) and apply the fastest. */
+int sdesim_batch_autotune(sdesim_batch* b, void* stream, int32_t* paths_per_lane, int32_t* block_size, double* best_ms);
+/* Eager run: counter reset, persistent kernel, per-instance finalize. Asynchronous. */
+int sdesim_batch_run(sdesim_batch* b, void* stream);
+/* Capture the three launches into a hipGraph (on a stream the batch owns) / replay it on `stream`. */
+int sdesim_batch_capture(sdesim_batch* b);
+int sdesim_batch_replay(sdesim_batch* b, void* stream);
+/* out[2i] = mean, out[2i+1] = sample variance of instance i's terminal values. Synchronises. */
+int sdesim_batch_moments(sdesim_batch* b, double* h_out, void* stream);
+/* Terminal values (f32 or f64 by precision), instance-major. Synchronises. */
+int sdesim_batch_terminal(sdesim_batch* b, void* h_out, void* stream);
+int sdesim_batch_info(const sdesim_batch* b, int32_t* paths_per_lane, int32_t* block_size, int32_t* grid,
+                      int32_t* n_items, int64_t* total_paths);
+
 const char* sdesim_last_error(void);
 int32_t sdesim_version(void);
 

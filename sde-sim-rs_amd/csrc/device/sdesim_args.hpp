@@ -41,3 +41,27 @@ struct SimArgs {
 
 #define SDE_OUT_FILTRATION 1
 #define SDE_OUT_TERMINAL 2
+
+// ------------------------------------------------------------------------------------------
+// Batched 1-D instances (batch_kernels.hip)
+// ------------------------------------------------------------------------------------------
+// One SDE instance of a batch (built-in model, run-time parameters, uniform grid).
+struct SdeInstance {
+  double p0, p1, p2;   // GBM: mu, sigma; OU / CIR: kappa, theta, sigma
+  double x0;
+  double dt, sdt;      // f64 grid
+  float dtf, sdtf;     // f32 grid
+  sde_u64 seed;        // path s uses seed + s
+  long long paths;
+  int kind;            // 0 GBM, 1 OU, 2 CIR
+  int steps;
+};
+
+// One work item: up to 64 * PPL paths of one instance.
+struct SdeItem {
+  int instance;
+  int chunk;           // paths [chunk * 64 * PPL, ...)
+  int slot;            // partial slot (instance-major order)
+  int pad;
+};
+

@@ -254,6 +254,11 @@ T* upload(const std::vector<T>& v) {
 
 }  // namespace
 
+int sdesim_set_error(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
 struct sdesim_plan {
   CompiledKernel kernel;
   std::string source;

@@ -8,6 +8,8 @@ extensions select the seed, precision, Sobol scrambling, cache semantics and out
 by callers that keep results on the device.
 """
 from ._native import PanicException
+from . import models
 from .api import Plan, Universe, parse_equations, simulate
+from .batch import Batch
 
-__all__ = ["simulate", "parse_equations", "Universe", "Plan", "PanicException"]
+__all__ = ["simulate", "parse_equations", "Universe", "Plan", "Batch", "models", "PanicException"]

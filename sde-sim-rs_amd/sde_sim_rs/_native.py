@@ -34,6 +34,20 @@ class Options(ctypes.Structure):
     ]
 
 
+class Instance(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("steps", ctypes.c_int32),
+        ("paths", ctypes.c_int64),
+        ("x0", ctypes.c_double),
+        ("horizon", ctypes.c_double),
+        ("p0", ctypes.c_double),
+        ("p1", ctypes.c_double),
+        ("p2", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/sdesim.h declares.
 SIGNATURES = {
     "sdesim_default_options": (None, [ctypes.POINTER(Options)]),
@@ -68,6 +82,20 @@ SIGNATURES = {
                                               ctypes.POINTER(ctypes.c_double)]),
     "sdesim_sobol_points": (ctypes.c_int, [ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
                                            ctypes.POINTER(ctypes.c_uint32)]),
+    "sdesim_batch_create": (ctypes.c_int, [ctypes.POINTER(Instance), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+                                           ctypes.POINTER(ctypes.c_void_p)]),
+    "sdesim_batch_free": (None, [ctypes.c_void_p]),
+    "sdesim_batch_set_config": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32]),
+    "sdesim_batch_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32),
+                                             ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_double)]),
+    "sdesim_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "sdesim_batch_capture": (ctypes.c_int, [ctypes.c_void_p]),
+    "sdesim_batch_replay": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "sdesim_batch_moments": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
+    "sdesim_batch_terminal": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
+    "sdesim_batch_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
+                                         ctypes.POINTER(ctypes.c_int64)]),
     "sdesim_last_error": (ctypes.c_char_p, []),
     "sdesim_version": (ctypes.c_int32, []),
 }
