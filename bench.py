@@ -56,6 +56,21 @@ def _time_plan(plan, out, steps, warmup):
     return wall, kern
 
 
+def _profiled_traffic(paths, nsteps):
+    """HBM bytes per launch of the headline kernel from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy); None for other sizes."""
+    if (paths, nsteps) != (1 << 20, 1024):
+        return None
+    path = os.path.join(ROOT, "profiles", "sdesim_rk_f32_sobolxor_filt_heston_2p20x1024.md")
+    try:
+        for ln in open(path):
+            if ln.startswith("traffic_bytes_per_launch:"):
+                return float(ln.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(paths, steps_n, threads):
     import numpy as np
 
@@ -82,7 +97,8 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--cpu-paths", type=int, default=1 << 17)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--only", default="", help="comma list of legs to run: f32,f64,term,batch (profiling)")
+    ap.add_argument("--calib", action="store_true", help="run a 1 GiB device copy first (HBM counter calibration)")
+    ap.add_argument("--only", default="", help="comma list of legs to run: f32,f64,term,batch,adjoint")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: paths are independent and the multi-GPU form "
@@ -96,11 +112,17 @@ def main():
 
     S, N = args.paths, args.nsteps
     times = np.linspace(0.0, 1.0, N + 1)
-    legs = set(args.only.split(",")) if args.only else {"f32", "f64", "term", "batch"}
+    legs = set(args.only.split(",")) if args.only else {"f32", "f64", "term", "batch", "adjoint"}
     common = dict(seed=0, scramble="xor", scramble_seed=7, paths_per_lane=args.ppl, block_size=args.block)
     uni = sde_sim_rs.Universe(HESTON)
     res = {}
 
+    if args.calib:
+        src = torch.empty(1 << 28, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     if "f32" in legs:
         p32 = sde_sim_rs.Plan(uni, times, S, INIT, "sobol", "runge-kutta", precision="f32", **common)
         f32 = p32.new_filtration()
@@ -147,6 +169,28 @@ def main():
                  "graph": True, "paths_per_lane": ppl, "block_size": blk, "grid": info["grid"],
                  "work_items": info["work_items"]}
 
+    adj = None
+    if "adjoint" in legs:
+        from sde_sim_rs.adjoint import Basket, basket_config
+        c = basket_config(seed=3)
+        AS, AN = 1 << 22, 365
+        bk = Basket(8, AN, AS, horizon=1.0, strike=100.0, width=0.5, seed=3, precision="f32")
+        times = {}
+        for mode in (False, True):
+            for _ in range(max(1, args.warmup)):
+                bk.evaluate(c["x0"], c["mu"], c["sigma"], c["rho"], adjoint=mode)
+            reps = max(1, args.steps // 4)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                r = bk.evaluate(c["x0"], c["mu"], c["sigma"], c["rho"], adjoint=mode)
+            times[mode] = (time.perf_counter() - t0) * 1e3 / reps
+        adj = {"workload": "8-asset correlated GBM basket, softplus call K=100 w=0.5, 365 Euler steps, 2^22 paths, f32",
+               "greeks": 52, "adjoint_ms": times[True], "price_ms": times[False],
+               "adjoint_over_price": times[True] / times[False],
+               "path_steps_per_s_adjoint": AS * AN / (times[True] * 1e-3), "price": r["price"],
+               "checkpoint_every": 19, "checkpoint_bytes": ((AN + 18) // 19) * 8 * AS * 4}
+        del bk
+
     path_steps = S * N
     line = {"metric": "path-steps/s on 1 MI355X, FP32 vs FP64 plain at equal error; HBM GB/s %peak",
             "unit": "path-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -161,8 +205,8 @@ def main():
         line.update({"value": path_steps / (wall * 1e-3), "ms_per_step": wall})
         ach = bytes_alg / (kern * 1e-3) / 1e9
         line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                            "kernel": "sdesim_kernel (f32, full filtration)", "kernel_ms": kern,
+                            "frac": ach / HBM_PEAK_GBS, "traffic": _profiled_traffic(S, N),
+                            "kernel": "sdesim_rk_f32_sobolxor_filt", "kernel_ms": kern,
                             "algorithmic_bytes": bytes_alg}
     if "f64" in res:
         wall, kern = res["f64"]
@@ -178,6 +222,8 @@ def main():
         line["terminal_only"] = term
     if batch:
         line["batched"] = batch
+    if adj:
+        line["adjoint"] = adj
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         v, dt = cpu_baseline(args.cpu_paths, N, threads)

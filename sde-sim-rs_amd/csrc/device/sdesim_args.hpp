@@ -65,3 +65,18 @@ struct SdeItem {
   int pad;
 };
 
+
+// ------------------------------------------------------------------------------------------
+// Correlated GBM basket, price and adjoint (basket_kernels.hip)
+// ------------------------------------------------------------------------------------------
+struct BasketArgs {
+  double x0[8], mu[8], sigma[8], weight[8];
+  double chol[64];            // lower Cholesky factor of the correlation, row-major 8 x 8
+  double dt, sdt, strike, width;
+  sde_u64 seed;               // path s uses seed + s
+  long long paths;
+  void* checkpoints;          // [slots][D][paths] real
+  double* partials;           // [gridDim.x][NQ]
+  int steps;
+  int pad_;
+};

@@ -48,6 +48,20 @@ class Instance(ctypes.Structure):
     ]
 
 
+class BasketSpec(ctypes.Structure):
+    _fields_ = [
+        ("assets", ctypes.c_int32),
+        ("steps", ctypes.c_int32),
+        ("paths", ctypes.c_int64),
+        ("horizon", ctypes.c_double),
+        ("strike", ctypes.c_double),
+        ("width", ctypes.c_double),
+        ("seed", ctypes.c_uint64),
+        ("precision", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/sdesim.h declares.
 SIGNATURES = {
     "sdesim_default_options": (None, [ctypes.POINTER(Options)]),
@@ -96,6 +110,10 @@ SIGNATURES = {
     "sdesim_batch_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                          ctypes.POINTER(ctypes.c_int64)]),
+    "sdesim_basket_create": (ctypes.c_int, [ctypes.POINTER(BasketSpec), ctypes.POINTER(ctypes.c_void_p)]),
+    "sdesim_basket_free": (None, [ctypes.c_void_p]),
+    "sdesim_basket_eval": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_double)] * 5 +
+                           [ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
     "sdesim_last_error": (ctypes.c_char_p, []),
     "sdesim_version": (ctypes.c_int32, []),
 }

@@ -1,0 +1,134 @@
+"""Basket price and pathwise adjoint on the GPU: against the DSL form, against reverse-mode
+autodiff of the same paths (torch, CPU float64), against central finite differences with
+common random numbers, and against Black-Scholes delta and vega for one asset."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+
+def _cfg():
+    from sde_sim_rs.adjoint import basket_config
+    return basket_config(seed=3)
+
+
+def test_basket_price_matches_dsl_form():
+    import sde_sim_rs
+    from sde_sim_rs.adjoint import Basket, basket_dsl
+    c = _cfg()
+    S, N = 1 << 14, 73
+    print("case: basket f64 price vs DSL form S=%d N=%d" % (S, N), flush=True)
+    b = Basket(8, N, S, horizon=1.0, strike=100.0, width=0.0, seed=17, precision="f64")
+    r = b.evaluate(c["x0"], c["mu"], c["sigma"], c["rho"], adjoint=False)
+    eqs, init = basket_dsl(c["x0"], c["mu"], c["sigma"], c["rho"])
+    times = np.arange(N + 1) * (1.0 / N)
+    plan = sde_sim_rs.Plan(eqs, times, S, init, "pseudo", "euler", seed=17, outputs=("terminal",),
+                           payoff="max((X0 + X1 + X2 + X3 + X4 + X5 + X6 + X7) / 8 - 100, 0)")
+    plan.run(None)
+    m = plan.moments()["payoff"]
+    assert abs(r["price"] - m[0]) <= 1e-9 * abs(m[0]), (r["price"], m[0])
+
+
+def test_adjoint_f64_matches_torch_autodiff(oracle):
+    from sde_sim_rs.adjoint import Basket
+    c = _cfg()
+    S, N, D, seed = 32, 45, 8, 1234
+    print("case: basket f64 adjoint vs torch autodiff S=%d N=%d" % (S, N), flush=True)
+    b = Basket(D, N, S, horizon=1.0, strike=100.0, width=0.5, seed=seed, precision="f64")
+    r = b.evaluate(c["x0"], c["mu"], c["sigma"], c["rho"], adjoint=True)
+    # Reference: the same paths in torch float64 on the CPU, differentiated by autograd.
+    L = torch.tensor(np.linalg.cholesky(c["rho"]), dtype=torch.float64, requires_grad=True)
+    x0 = torch.tensor(c["x0"], requires_grad=True)
+    mu = torch.tensor(c["mu"], requires_grad=True)
+    sg = torch.tensor(c["sigma"], requires_grad=True)
+    eps = np.zeros((S, N, D))
+    for s in range(S):
+        u = oracle.pseudo_f64(seed + s, N * D)
+        eps[s] = np.array([oracle.inv_normal(v) for v in u]).reshape(N, D)
+    eps = torch.tensor(eps)
+    dt = 1.0 / N
+    X = x0.expand(S, D)
+    for n in range(N):
+        Z = eps[:, n, :] @ torch.tril(L).T
+        X = X + X * (mu * dt + sg * np.sqrt(dt) * Z)
+    B = X.mean(dim=1)
+    pay = 0.5 * torch.nn.functional.softplus((B - 100.0) / 0.5)
+    pay.mean().backward()
+    assert abs(r["price"] - float(pay.mean())) <= 1e-12 * abs(r["price"])
+    for name, t in (("x0", x0), ("mu", mu), ("sigma", sg)):
+        np.testing.assert_allclose(r[name], t.grad.numpy(), rtol=1e-9, atol=1e-12, err_msg=name)
+    np.testing.assert_allclose(r["chol"], np.tril(L.grad.numpy()), rtol=1e-9, atol=1e-12)
+
+
+def test_adjoint_f32_against_central_differences():
+    from sde_sim_rs.adjoint import Basket
+    c = _cfg()
+    S, N = 1 << 16, 365
+    print("case: basket f32 adjoint vs f64 central FD (CRN) S=%d N=%d" % (S, N), flush=True)
+    adj = Basket(8, N, S, strike=100.0, width=0.5, seed=99, precision="f32").evaluate(
+        c["x0"], c["mu"], c["sigma"], c["rho"], adjoint=True)
+    fwd = Basket(8, N, S, strike=100.0, width=0.5, seed=99, precision="f64")
+    price = lambda **kw: fwd.evaluate(**{**c, **kw}, adjoint=False)["price"]  # noqa: E731
+    worst = 0.0
+    for name in ("x0", "mu", "sigma"):
+        for i in range(8):
+            v = c[name].copy()
+            h = 1e-3 * abs(v[i])
+            vp, vm = v.copy(), v.copy()
+            vp[i] += h
+            vm[i] -= h
+            fd = (price(**{name: vp}) - price(**{name: vm})) / (2 * h)
+            scale = max(abs(fd), 1e-3 * np.max(np.abs(adj[name])))
+            err = abs(adj[name][i] - fd) / scale
+            worst = max(worst, err)
+            assert err <= 1e-4, (name, i, adj[name][i], fd)
+    rho = c["rho"]
+    for i in range(8):
+        for j in range(i):
+            h = 1e-3 * max(abs(rho[i, j]), 0.1)
+            rp, rm = rho.copy(), rho.copy()
+            rp[i, j] = rp[j, i] = rho[i, j] + h
+            rm[i, j] = rm[j, i] = rho[i, j] - h
+            fd = (price(rho=rp) - price(rho=rm)) / (2 * h)
+            an = adj["rho"][i, j] + adj["rho"][j, i]
+            scale = max(abs(fd), 1e-3 * np.max(np.abs(adj["rho"])))
+            err = abs(an - fd) / scale
+            worst = max(worst, err)
+            assert err <= 1e-4, ("rho", i, j, an, fd)
+    print("max relative error of 52 adjoint Greeks vs FD: %.3g" % worst)
+
+
+def test_black_scholes_delta_and_vega_one_asset():
+    from scipy.stats import norm
+    from sde_sim_rs.adjoint import Basket
+    x0, mu, sig, K, T = 100.0, 0.03, 0.25, 100.0, 1.0
+    S, N = 1 << 20, 365
+    print("case: one-asset call delta/vega vs Black-Scholes S=%d N=%d" % (S, N), flush=True)
+    r = Basket(1, N, S, horizon=T, strike=K, width=0.0, seed=5, precision="f32").evaluate(
+        [x0], [mu], [sig], [[1.0]], adjoint=True)
+    F = x0 * np.exp(mu * T)
+    d1 = (np.log(x0 / K) + (mu + 0.5 * sig * sig) * T) / (sig * np.sqrt(T))
+    price = F * norm.cdf(d1) - K * norm.cdf(d1 - sig * np.sqrt(T))
+    delta = np.exp(mu * T) * norm.cdf(d1)
+    vega = F * norm.pdf(d1) * np.sqrt(T)
+    se = np.sqrt(r["variance"] / S)
+    assert abs(r["price"] - price) < 4 * se + 2e-3 * price, (r["price"], price)
+    assert abs(r["x0"][0] - delta) < 4e-3 * delta, (r["x0"][0], delta)
+    assert abs(r["sigma"][0] - vega) < 1.5e-2 * vega, (r["sigma"][0], vega)
+
+
+def test_torch_autograd_function():
+    from sde_sim_rs.adjoint import Basket, basket_price
+    c = _cfg()
+    b = Basket(8, 50, 1 << 14, seed=3, precision="f32")
+    x0 = torch.tensor(c["x0"], requires_grad=True)
+    mu = torch.tensor(c["mu"], requires_grad=True)
+    sg = torch.tensor(c["sigma"], requires_grad=True)
+    rho = torch.tensor(c["rho"], requires_grad=True)
+    p = basket_price(x0, mu, sg, rho, b)
+    (2.0 * p).backward()
+    r = b.evaluate(c["x0"], c["mu"], c["sigma"], c["rho"])
+    np.testing.assert_allclose(x0.grad.numpy(), 2 * r["x0"], rtol=1e-12)
+    np.testing.assert_allclose(rho.grad.numpy(), 2 * r["rho"], rtol=1e-12)

@@ -58,9 +58,10 @@ def test_batch_moments_fp32_vs_fp64_and_closed_form():
         mean32 = m["f32"][i][0]
         se = np.sqrt(var64 / inst["paths"])
         if inst["kind"] == models.CIR:
-            # Truncated CIR near zero amplifies f32 rounding through sqrt (a CPU float32/float64
-            # emulation of instance 104 shows a 1e-4 relative gap); bound it against the MC error.
-            assert abs(mean32 - mean64) <= 0.1 * se + 1e-12, (i, inst, mean32, mean64, se)
+            # sigma * sqrt(max(X, 0)) is not Lipschitz at 0: once a path comes near zero its f32
+            # and f64 copies decorrelate, so the two means are two correlated MC estimates of the
+            # same value and the bound is statistical.
+            assert abs(mean32 - mean64) <= 2.0 * se + 1e-12, (i, inst, mean32, mean64, se)
         else:
             assert abs(mean32 - mean64) <= 1e-5 * abs(mean64), (i, inst, mean32, mean64)
         if inst["kind"] in (models.GBM, models.OU):
