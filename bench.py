@@ -98,7 +98,7 @@ def main():
     ap.add_argument("--cpu-paths", type=int, default=1 << 17)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true", help="run a 1 GiB device copy first (HBM counter calibration)")
-    ap.add_argument("--only", default="", help="comma list of legs to run: f32,f64,term,batch,adjoint")
+    ap.add_argument("--only", default="", help="comma list of legs to run: f32,f64,term,batch,adjoint,lmm")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: paths are independent and the multi-GPU form "
@@ -112,7 +112,7 @@ def main():
 
     S, N = args.paths, args.nsteps
     times = np.linspace(0.0, 1.0, N + 1)
-    legs = set(args.only.split(",")) if args.only else {"f32", "f64", "term", "batch", "adjoint"}
+    legs = set(args.only.split(",")) if args.only else {"f32", "f64", "term", "batch", "adjoint", "lmm"}
     common = dict(seed=0, scramble="xor", scramble_seed=7, paths_per_lane=args.ppl, block_size=args.block)
     uni = sde_sim_rs.Universe(HESTON)
     res = {}
@@ -191,6 +191,27 @@ def main():
                "checkpoint_every": 19, "checkpoint_bytes": ((AN + 18) // 19) * 8 * AS * 4}
         del bk
 
+    lmm = None
+    if "lmm" in legs:
+        from sde_sim_rs.lmm import ForwardRateModel, forward_rate_config
+        c = forward_rate_config(seed=5)
+        LS, LN = 1 << 20, 160
+        fm = ForwardRateModel(steps=LN, paths=LS, expiry_index=4, strike=0.03, scramble_seed=5, precision="f32")
+        times = {}
+        for mode in (False, True):
+            for _ in range(max(1, args.warmup)):
+                fm.evaluate(c["L0"], c["vol"], c["loadings"], adjoint=mode)
+            reps = max(1, args.steps // 4)
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                r = fm.evaluate(c["L0"], c["vol"], c["loadings"], adjoint=mode)
+            times[mode] = (time.perf_counter() - t0) * 1e3 / reps
+        lmm = {"workload": "40 forwards, 3 factors, 1y x 9y receiver swaption K=0.03, 160 log-Euler steps, "
+                           "2^20 paths, Sobol-XOR dim 480, f32",
+               "sensitivities": 40, "adjoint_ms": times[True], "price_ms": times[False],
+               "forward_steps_per_s_price": LS * LN * 40 / (times[False] * 1e-3), "price": r["price"]}
+        del fm
+
     path_steps = S * N
     line = {"metric": "path-steps/s on 1 MI355X, FP32 vs FP64 plain at equal error; HBM GB/s %peak",
             "unit": "path-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -224,6 +245,8 @@ def main():
         line["batched"] = batch
     if adj:
         line["adjoint"] = adj
+    if lmm:
+        line["forward_rate_model"] = lmm
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         v, dt = cpu_baseline(args.cpu_paths, N, threads)

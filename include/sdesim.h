@@ -212,6 +212,36 @@ void sdesim_basket_free(sdesim_basket* b);
 int sdesim_basket_eval(sdesim_basket* b, const double* x0, const double* mu, const double* sigma,
                        const double* weight, const double* chol, int32_t adjoint, double* out, void* stream);
 
+/* ---------------------------------------------------------------------------------------
+ * Forward-rate (LIBOR market) model: receiver swaption price and the 40 curve sensitivities.
+ * No reference counterpart. 40 forwards on tenor `tenor`, 3 factors (loadings [40][3], unit
+ * rows), constant volatilities; spot-measure log-Euler with the drift as a prefix sum over
+ * forwards; `steps` = steps_per_period * expiry_index steps to expiry T_e; uniforms from the
+ * Sobol sequence (dimension steps * 3, point index = path) with a digital XOR shift drawn from
+ * scramble_seed. Payoff max(K A - (1 - P_last), 0) / numeraire (swap over [T_e, T_40]).
+ * --------------------------------------------------------------------------------------- */
+typedef struct sdesim_lmm sdesim_lmm;
+
+typedef struct sdesim_lmm_spec {
+  int32_t forwards;          /* 40 */
+  int32_t factors;           /* 3 */
+  int32_t steps;
+  int32_t steps_per_period;
+  int32_t expiry_index;      /* e: the swaption expires at T_e = e * tenor */
+  int32_t precision;         /* SDESIM_F32 | SDESIM_F64 */
+  int64_t paths;             /* <= 2^22 */
+  double tenor;
+  double strike;
+  uint64_t scramble_seed;
+} sdesim_lmm_spec;
+
+int sdesim_lmm_create(const sdesim_lmm_spec* spec, sdesim_lmm** out);
+void sdesim_lmm_free(sdesim_lmm* m);
+/* out[0] = price, out[1] = sample variance of the discounted payoff, and with adjoint != 0
+ * out[2 + i] = d price / d L_i(0), i = 0..39. Synchronises the stream. */
+int sdesim_lmm_eval(sdesim_lmm* m, const double* L0, const double* vol, const double* loadings, int32_t adjoint,
+                    double* out, void* stream);
+
 const char* sdesim_last_error(void);
 int32_t sdesim_version(void);
 

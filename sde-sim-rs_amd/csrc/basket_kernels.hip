@@ -263,3 +263,8 @@ hipError_t sdesim_launch_basket(int f32, int assets, int adjoint, int grid, cons
   hipLaunchKernelGGL(sdesim_sum_partials, dim3(nq), dim3(256), 0, stream, a.partials, grid, nq, sums);
   return hipGetLastError();
 }
+
+hipError_t sdesim_launch_sum_partials(const double* partials, int nblocks, int nq, double* out, hipStream_t stream) {
+  hipLaunchKernelGGL(sdesim_sum_partials, dim3(nq), dim3(256), 0, stream, partials, nblocks, nq, out);
+  return hipGetLastError();
+}

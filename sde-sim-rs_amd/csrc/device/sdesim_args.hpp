@@ -80,3 +80,23 @@ struct BasketArgs {
   int steps;
   int pad_;
 };
+
+// ------------------------------------------------------------------------------------------
+// Forward-rate model, price and curve sensitivities (lmm_kernels.hip); 40 forwards, 3 factors
+// ------------------------------------------------------------------------------------------
+struct LmmArgs {
+  double L0[40], vol[40], load[120];  // initial forwards, volatilities, loadings [40][3]
+  float volf[40], loadf[120];         // f32 copies for f32 kernels
+  double dt, sdt, tau, strike;
+  long long paths;
+  const sde_u32* sobol_tab;           // sliced Sobol tables (3 slices)
+  const sde_u32* xor_mask;            // [steps * 3]
+  void* checkpoints;                  // [slots][40][paths] real
+  double* partials;                   // [gridDim.x][42]
+  int steps;
+  int steps_per_period;
+  int expiry_index;
+  int sobol_tab_words;
+  int zero;                           // always 0: makes per-step parameter reads step-dependent
+  int pad_;
+};

@@ -1,0 +1,263 @@
+// lmm_kernels.hip -- multi-factor forward-rate (LIBOR market) model: swaption price and pathwise
+// curve sensitivities. No reference counterpart (the reference has no built-in models and no
+// sensitivities); the stepping is log-Euler with uniforms from the same scrambled Sobol stream
+// as the generic kernel (src/rng/sobol.rs points, XOR digital shift) and the A&S inverse normal
+// (src/proc/increment.rs:160-179).
+//
+// Forwards L_i on [T_i, T_{i+1}], T_i = i * tau, i = 0 .. NF-1, NF = 40; F = 3 factors with
+// loadings B (NF x F, unit rows); volatilities sig_i. Spot measure: at step n (t_n = n dt) the
+// alive forwards are i >= eta = n / steps_per_period + 1, and for them
+//   ln L_i += (sig_i sum_f B_if C_f(i) - sig_i^2 / 2) dt + sig_i sqrt(dt) sum_f B_if eps_f,
+//   C_f(i) = sum_{j=eta..i} tau sig_j L_j / (1 + tau L_j) B_jf     (a prefix sum over j),
+// eps_f = Psi(u_{n F + f}). Dead forwards keep their fixing value.
+// Payoff at expiry T_e (e = expiry index): receiver swaption on the swap over [T_e, T_NF] with
+// fixed rate K: V = max(K A - (1 - P_{NF-1}), 0) / N, P_j = prod_{k=e..j} 1 / (1 + tau L_k),
+// A = tau sum_{j=e..NF-1} P_j, numeraire N = prod_{k<e} (1 + tau L_k).
+//
+// Adjoint: the forward pass stores L every SEG steps (checkpoints in HBM, [slot][i][path]); the
+// reverse sweep recomputes each segment from its checkpoint and steps back through it. For one
+// step, with a_j = tau sig_j L_j / (1 + tau L_j), E_i = L'_i / L_i:
+//   Cbar_f(i) = Lbar'_i L'_i dt sig_i B_if,   G_f(j) = sum_{i >= j} Cbar_f(i)  (a suffix sum),
+//   Lbar_j = Lbar'_j E_j + (sum_f B_jf G_f(j)) tau sig_j / (1 + tau L_j)^2.
+#include <hip/hip_runtime.h>
+
+#include "device/sdesim_device.hpp"
+
+namespace {
+
+constexpr int NF = 40;
+constexpr int F = 3;
+constexpr int SEG = 8;
+constexpr int NSL = 3;  // Sobol table slices: point indices below 2^22
+
+template <typename R>
+struct LmmStep {
+  R eps[F];
+};
+
+template <typename R>
+__device__ __forceinline__ R lmm_exp(R x) {
+  if constexpr (sizeof(R) == 4) return __expf(x);
+  else return exp(x);
+}
+
+// The F uniforms of step n from the Sobol tables (digital XOR shift), as standard normals.
+template <typename R>
+__device__ __forceinline__ void lmm_noise(const LmmArgs& a, __amdgpu_buffer_rsrc_t rsrc, const sde_u32 (&soff)[NSL],
+                                          int n, R (&eps)[F]) {
+  const sde_u32 stride = 64u + 256u * (NSL - 1);
+#pragma unroll
+  for (int f = 0; f < F; ++f) {
+    const int d = n * F + f;
+    const int base = (int)((sde_u32)d * stride * 4u);
+    sde_u32 x = a.xor_mask[d];
+#pragma unroll
+    for (int sl = 0; sl < NSL; ++sl) x ^= (sde_u32)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)soff[sl], base, 0);
+    SdeUnifX u;
+    u.y = x;
+    eps[f] = sde_psi<R>(u);
+  }
+}
+
+// Model parameters live in the kernel-argument block. Each step reads them at index i + off,
+// where off = step * a.zero is 0 at run time but not known to the compiler, so the reads are
+// uniform scalar loads repeated per step instead of 160 loop-invariant values hoisted into
+// vector registers for the whole kernel.
+template <typename R>
+struct LmmPar {
+  const LmmArgs& a;
+  int off;
+  __device__ __forceinline__ R vol(int i) const {
+    if constexpr (sizeof(R) == 4) return a.volf[i + off];
+    else return a.vol[i + off];
+  }
+  __device__ __forceinline__ R load(int i, int f) const {
+    if constexpr (sizeof(R) == 4) return a.loadf[(i + off) * F + f];
+    else return a.load[(i + off) * F + f];
+  }
+};
+
+// One log-Euler step of all alive forwards (forwards below `eta` are dead and unchanged).
+template <typename R>
+__device__ __forceinline__ void lmm_step(const LmmPar<R>& tab, R (&L)[NF], const R (&eps)[F], int eta, R dt, R sdt,
+                                         R tau) {
+  R C[F] = {R(0), R(0), R(0)};
+#pragma unroll
+  for (int i = 0; i < NF; ++i) {
+    if (i >= eta) {
+      const R sig = tab.vol(i);
+      const R b0 = tab.load(i, 0), b1 = tab.load(i, 1), b2 = tab.load(i, 2);
+      const R ai = tau * sig * L[i] / (R(1) + tau * L[i]);
+      C[0] = C[0] + ai * b0;
+      C[1] = C[1] + ai * b1;
+      C[2] = C[2] + ai * b2;
+      const R drift = sig * (b0 * C[0] + b1 * C[1] + b2 * C[2]) - R(0.5) * sig * sig;
+      const R w = b0 * eps[0] + b1 * eps[1] + b2 * eps[2];
+      L[i] = L[i] * lmm_exp<R>(drift * dt + sig * sdt * w);
+    }
+  }
+}
+
+// Adjoint of lmm_step: on entry L is the state before the step and Lb the adjoint of the state
+// after it; on exit Lb is the adjoint of the state before it.
+template <typename R>
+__device__ __forceinline__ void lmm_step_adjoint(const LmmPar<R>& tab, const R (&L)[NF], R (&Lb)[NF],
+                                                 const R (&eps)[F], int eta, R dt, R sdt, R tau) {
+  R E[NF], Lp[NF];
+  R C[F] = {R(0), R(0), R(0)};
+#pragma unroll
+  for (int i = 0; i < NF; ++i) {
+    E[i] = R(1);
+    Lp[i] = L[i];
+    if (i >= eta) {
+      const R sig = tab.vol(i);
+      const R b0 = tab.load(i, 0), b1 = tab.load(i, 1), b2 = tab.load(i, 2);
+      const R ai = tau * sig * L[i] / (R(1) + tau * L[i]);
+      C[0] = C[0] + ai * b0;
+      C[1] = C[1] + ai * b1;
+      C[2] = C[2] + ai * b2;
+      const R drift = sig * (b0 * C[0] + b1 * C[1] + b2 * C[2]) - R(0.5) * sig * sig;
+      const R w = b0 * eps[0] + b1 * eps[1] + b2 * eps[2];
+      E[i] = lmm_exp<R>(drift * dt + sig * sdt * w);
+      Lp[i] = L[i] * E[i];
+    }
+  }
+  R G[F] = {R(0), R(0), R(0)};
+#pragma unroll
+  for (int i = NF - 1; i >= 0; --i) {
+    if (i >= eta) {
+      const R sig = tab.vol(i);
+      const R b0 = tab.load(i, 0), b1 = tab.load(i, 1), b2 = tab.load(i, 2);
+      const R db = Lb[i] * Lp[i] * dt;
+      G[0] = G[0] + db * sig * b0;
+      G[1] = G[1] + db * sig * b1;
+      G[2] = G[2] + db * sig * b2;
+      const R ab = b0 * G[0] + b1 * G[1] + b2 * G[2];
+      const R op = R(1) + tau * L[i];
+      Lb[i] = Lb[i] * E[i] + ab * tau * sig / (op * op);
+    }
+  }
+}
+
+template <typename R, bool ADJ>
+__global__ void __launch_bounds__(256) sdesim_lmm(const LmmArgs a) {
+  constexpr int NQ = 2 + NF;
+  const long long S = a.paths;
+  const int N = a.steps;
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = s < S;
+  const long long sid = live ? s : 0;
+  const R dt = (R)a.dt, sdt = (R)a.sdt, tau = (R)a.tau, K = (R)a.strike;
+  const int e = a.expiry_index;
+
+  sde_u32 soff[NSL];
+  {
+    const sde_u64 g = (sde_u64)sid ^ ((sde_u64)sid >> 1);
+    sde_sobol_offsets<NSL>(g, soff);
+#pragma unroll
+    for (int sl = 0; sl < NSL; ++sl) soff[sl] *= 4u;
+  }
+  __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc((void*)a.sobol_tab, (short)0, a.sobol_tab_words * 4, 0x00020000);
+
+  R L[NF];
+#pragma unroll
+  for (int i = 0; i < NF; ++i) L[i] = (R)a.L0[i];
+  R* ck = (R*)a.checkpoints;
+  for (int n = 0; n < N; ++n) {
+    if (ADJ && (n % SEG) == 0 && live) {
+      const long long slot = n / SEG;
+#pragma unroll
+      for (int i = 0; i < NF; ++i) ck[(slot * NF + i) * S + sid] = L[i];
+    }
+    R eps[F];
+    lmm_noise<R>(a, rsrc, soff, n, eps);
+    lmm_step<R>(LmmPar<R>{a, n * a.zero}, L, eps, n / a.steps_per_period + 1, dt, sdt, tau);
+  }
+
+  // Payoff and its derivative with respect to the terminal forwards.
+  R numer = R(1);
+#pragma unroll
+  for (int k = 0; k < NF; ++k)
+    if (k < e) numer = numer * (R(1) + tau * L[k]);
+  R P = R(1), A = R(0);
+  R Pj[NF];
+#pragma unroll
+  for (int j = 0; j < NF; ++j) {
+    Pj[j] = R(0);
+    if (j >= e) {
+      P = P / (R(1) + tau * L[j]);
+      Pj[j] = P;
+      A = A + tau * P;
+    }
+  }
+  const R U = K * A - (R(1) - P);
+  const R V = U > R(0) ? U / numer : R(0);
+
+  double acc[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
+  if (live) {
+    acc[0] = (double)V;
+    acc[1] = (double)V * (double)V;
+  }
+  if (ADJ) {
+    R Lb[NF];
+    // dV/dL_k: numeraire part for k < e; swap part for k >= e (only when in the money).
+    R tail = R(0);  // sum_{j >= k} P_j, built from the end
+#pragma unroll
+    for (int k = NF - 1; k >= 0; --k) {
+      if (k >= e) tail = tail + Pj[k];
+      const R op = R(1) + tau * L[k];
+      if (k < e) {
+        Lb[k] = -V * tau / op;
+      } else {
+        Lb[k] = U > R(0) ? (-K * tau * tau * tail - tau * P) / (op * numer) : R(0);
+      }
+    }
+    const int nseg = (N + SEG - 1) / SEG;
+    for (int seg = nseg - 1; seg >= 0; --seg) {
+      const int n0 = seg * SEG;
+      R Xs[SEG][NF];
+      R Xc[NF];
+#pragma unroll
+      for (int i = 0; i < NF; ++i) Xc[i] = ck[((long long)seg * NF + i) * S + sid];
+      for (int k = 0; k < SEG; ++k) {
+#pragma unroll
+        for (int i = 0; i < NF; ++i) Xs[k][i] = Xc[i];
+        if (n0 + k < N && k + 1 < SEG) {
+          R eps[F];
+          lmm_noise<R>(a, rsrc, soff, n0 + k, eps);
+          lmm_step<R>(LmmPar<R>{a, (n0 + k) * a.zero}, Xc, eps, (n0 + k) / a.steps_per_period + 1, dt, sdt,
+                      tau);
+        }
+      }
+      for (int k = SEG - 1; k >= 0; --k) {
+        if (n0 + k < N) {
+          R eps[F];
+          lmm_noise<R>(a, rsrc, soff, n0 + k, eps);
+          R Lk[NF];
+#pragma unroll
+          for (int i = 0; i < NF; ++i) Lk[i] = Xs[k][i];
+          lmm_step_adjoint<R>(LmmPar<R>{a, (n0 + k) * a.zero}, Lk, Lb, eps, (n0 + k) / a.steps_per_period + 1, dt,
+                              sdt, tau);
+        }
+      }
+    }
+    if (live) {
+#pragma unroll
+      for (int i = 0; i < NF; ++i) acc[2 + i] = (double)Lb[i];
+    }
+  }
+  sde_block_reduce_store<NQ>(acc, a.partials + (long long)blockIdx.x * NQ);
+}
+
+}  // namespace
+
+hipError_t sdesim_launch_lmm(int f32, int adjoint, int grid, const LmmArgs& a, hipStream_t stream) {
+  if (f32 && adjoint) hipLaunchKernelGGL((sdesim_lmm<float, true>), dim3(grid), dim3(256), 0, stream, a);
+  else if (f32) hipLaunchKernelGGL((sdesim_lmm<float, false>), dim3(grid), dim3(256), 0, stream, a);
+  else if (adjoint) hipLaunchKernelGGL((sdesim_lmm<double, true>), dim3(grid), dim3(256), 0, stream, a);
+  else hipLaunchKernelGGL((sdesim_lmm<double, false>), dim3(grid), dim3(256), 0, stream, a);
+  return hipGetLastError();
+}

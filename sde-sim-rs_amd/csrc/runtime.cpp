@@ -259,6 +259,17 @@ int sdesim_set_error(int code, const std::string& msg) {
   return code;
 }
 
+// Shared with the other runtime files: device Sobol tables and ChaCha8 words.
+const unsigned int* sdesim_device_sobol_tables(int dims, int nslices, int* words) {
+  SobolTables t = sobol_tables(dims, nslices);
+  if (words) *words = t.dims * (64 + 256 * (nslices - 1));
+  return t.dev;
+}
+
+void sdesim_chacha8_words(unsigned long long seed, size_t n, std::vector<unsigned int>& out) {
+  chacha8_words(seed, n, out);
+}
+
 struct sdesim_plan {
   CompiledKernel kernel;
   std::string source;

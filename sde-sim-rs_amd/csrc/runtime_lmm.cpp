@@ -1,0 +1,135 @@
+// runtime_lmm.cpp -- host side of the forward-rate model kernels (include/sdesim.h: sdesim_lmm_*).
+// No reference counterpart: the reference has no built-in models and computes no sensitivities.
+#include <hip/hip_runtime_api.h>
+
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/sdesim.h"
+#include "device/sdesim_args.hpp"
+
+hipError_t sdesim_launch_lmm(int f32, int adjoint, int grid, const LmmArgs& a, hipStream_t stream);
+hipError_t sdesim_launch_sum_partials(const double* partials, int nblocks, int nq, double* out, hipStream_t stream);
+int sdesim_set_error(int code, const std::string& msg);
+const unsigned int* sdesim_device_sobol_tables(int dims, int nslices, int* words);
+void sdesim_chacha8_words(unsigned long long seed, size_t n, std::vector<unsigned int>& out);
+
+namespace {
+constexpr int kNF = 40, kF = 3, kSeg = 8, kNQ = 2 + kNF;
+
+struct LFail {
+  int code;
+  std::string msg;
+};
+void lk(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw LFail{SDESIM_ERUNTIME, std::string(what) + ": " + hipGetErrorString(e)};
+}
+template <class F>
+int guard(F&& f) {
+  try {
+    f();
+    return SDESIM_OK;
+  } catch (const LFail& e) {
+    return sdesim_set_error(e.code, e.msg);
+  } catch (const std::exception& e) {
+    return sdesim_set_error(SDESIM_ERUNTIME, e.what());
+  }
+}
+}  // namespace
+
+struct sdesim_lmm {
+  sdesim_lmm_spec spec{};
+  long long grid = 0;
+  const unsigned int* tab = nullptr;
+  int tab_words = 0;
+  unsigned int* masks = nullptr;
+  void* ckpt = nullptr;
+  double* partials = nullptr;
+  double* sums = nullptr;
+  ~sdesim_lmm() {
+    for (void* p : {(void*)masks, ckpt, (void*)partials, (void*)sums})
+      if (p) (void)hipFree(p);
+  }
+};
+
+extern "C" {
+
+int sdesim_lmm_create(const sdesim_lmm_spec* spec, sdesim_lmm** out) {
+  return guard([&] {
+    if (!spec || !out) throw LFail{SDESIM_EVALUE, "null argument"};
+    const sdesim_lmm_spec& s = *spec;
+    if (s.forwards != kNF || s.factors != kF) throw LFail{SDESIM_EVALUE, "compiled for 40 forwards and 3 factors"};
+    if (s.steps < 1 || s.steps_per_period < 1) throw LFail{SDESIM_EVALUE, "steps must be >= 1"};
+    if (s.expiry_index < 1 || s.expiry_index >= kNF) throw LFail{SDESIM_EVALUE, "expiry_index out of range"};
+    if ((long long)s.steps_per_period * s.expiry_index != s.steps)
+      throw LFail{SDESIM_EVALUE, "steps must equal steps_per_period * expiry_index"};
+    if (s.paths < 1 || s.paths > (1LL << 22)) throw LFail{SDESIM_EVALUE, "paths must be in [1, 2^22]"};
+    if (!(s.tenor > 0.0)) throw LFail{SDESIM_EVALUE, "tenor must be > 0"};
+    std::unique_ptr<sdesim_lmm> m(new sdesim_lmm());
+    m->spec = s;
+    m->grid = (s.paths + 255) / 256;
+    const int dims = s.steps * kF;
+    m->tab = sdesim_device_sobol_tables(dims, 3, &m->tab_words);
+    std::vector<unsigned int> masks;
+    sdesim_chacha8_words(s.scramble_seed, (size_t)dims, masks);
+    lk(hipMalloc(&m->masks, sizeof(unsigned int) * dims), "hipMalloc(masks)");
+    lk(hipMemcpy(m->masks, masks.data(), sizeof(unsigned int) * dims, hipMemcpyHostToDevice), "hipMemcpy(masks)");
+    const size_t rb = s.precision == SDESIM_F32 ? 4 : 8;
+    const size_t slots = (size_t)(s.steps + kSeg - 1) / kSeg;
+    lk(hipMalloc(&m->ckpt, slots * kNF * (size_t)s.paths * rb), "hipMalloc(checkpoints)");
+    lk(hipMalloc(&m->partials, sizeof(double) * kNQ * m->grid), "hipMalloc(partials)");
+    lk(hipMalloc(&m->sums, sizeof(double) * kNQ), "hipMalloc(sums)");
+    *out = m.release();
+  });
+}
+
+void sdesim_lmm_free(sdesim_lmm* m) { delete m; }
+
+int sdesim_lmm_eval(sdesim_lmm* m, const double* L0, const double* vol, const double* loadings, int32_t adjoint,
+                    double* out, void* stream) {
+  return guard([&] {
+    if (!m || !L0 || !vol || !loadings || !out) throw LFail{SDESIM_EVALUE, "null argument"};
+    const sdesim_lmm_spec& s = m->spec;
+    LmmArgs a;
+    std::memset(&a, 0, sizeof a);
+    for (int i = 0; i < kNF; ++i) {
+      a.L0[i] = L0[i];
+      a.vol[i] = vol[i];
+      a.volf[i] = (float)vol[i];
+      for (int f = 0; f < kF; ++f) {
+        a.load[i * kF + f] = loadings[i * kF + f];
+        a.loadf[i * kF + f] = (float)loadings[i * kF + f];
+      }
+    }
+    const double T = s.tenor * s.expiry_index;
+    a.dt = T / s.steps;
+    a.sdt = std::sqrt(a.dt);
+    a.tau = s.tenor;
+    a.strike = s.strike;
+    a.paths = s.paths;
+    a.sobol_tab = m->tab;
+    a.sobol_tab_words = m->tab_words;
+    a.xor_mask = m->masks;
+    a.checkpoints = m->ckpt;
+    a.partials = m->partials;
+    a.steps = s.steps;
+    a.steps_per_period = s.steps_per_period;
+    a.expiry_index = s.expiry_index;
+    hipStream_t st = (hipStream_t)stream;
+    lk(sdesim_launch_lmm(s.precision == SDESIM_F32 ? 1 : 0, adjoint ? 1 : 0, (int)m->grid, a, st), "lmm launch");
+    lk(sdesim_launch_sum_partials(m->partials, (int)m->grid, kNQ, m->sums, st), "lmm reduce");
+    std::vector<double> sums(kNQ);
+    lk(hipMemcpyAsync(sums.data(), m->sums, sizeof(double) * kNQ, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+    lk(hipStreamSynchronize(st), "hipStreamSynchronize");
+    const double n = (double)s.paths;
+    out[0] = sums[0] / n;
+    out[1] = s.paths > 1 ? (sums[1] - sums[0] * out[0]) / (n - 1.0) : 0.0;
+    if (adjoint)
+      for (int i = 0; i < kNF; ++i) out[2 + i] = sums[2 + i] / n;
+  });
+}
+
+}  // extern "C"

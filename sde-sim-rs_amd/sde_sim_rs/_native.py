@@ -62,6 +62,21 @@ class BasketSpec(ctypes.Structure):
     ]
 
 
+class LmmSpec(ctypes.Structure):
+    _fields_ = [
+        ("forwards", ctypes.c_int32),
+        ("factors", ctypes.c_int32),
+        ("steps", ctypes.c_int32),
+        ("steps_per_period", ctypes.c_int32),
+        ("expiry_index", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+        ("paths", ctypes.c_int64),
+        ("tenor", ctypes.c_double),
+        ("strike", ctypes.c_double),
+        ("scramble_seed", ctypes.c_uint64),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/sdesim.h declares.
 SIGNATURES = {
     "sdesim_default_options": (None, [ctypes.POINTER(Options)]),
@@ -114,6 +129,10 @@ SIGNATURES = {
     "sdesim_basket_free": (None, [ctypes.c_void_p]),
     "sdesim_basket_eval": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_double)] * 5 +
                            [ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
+    "sdesim_lmm_create": (ctypes.c_int, [ctypes.POINTER(LmmSpec), ctypes.POINTER(ctypes.c_void_p)]),
+    "sdesim_lmm_free": (None, [ctypes.c_void_p]),
+    "sdesim_lmm_eval": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_double)] * 3 +
+                        [ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
     "sdesim_last_error": (ctypes.c_char_p, []),
     "sdesim_version": (ctypes.c_int32, []),
 }
