@@ -274,6 +274,8 @@ struct SdeChunk {
   static constexpr int BLOCKS = (M == 0) ? 1 : (U * M) / 8;
 };
 
+typedef sde_u32 SdeU32x2 __attribute__((ext_vector_type(2)));
+
 template <int RNG> struct SdeUnifOf { typedef double type; };
 template <> struct SdeUnifOf<SDE_RNG_SOBOL_XOR> { typedef SdeUnifX type; };
 
@@ -325,7 +327,6 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
   sde_u32 soff[PPL][NSL];  // byte offsets of this path's entries within a dimension's table row
   long long sid[PPL];
   bool live[PPL];
-  real* row[PPL];
 
 #pragma unroll
   for (int j = 0; j < PPL; ++j) {
@@ -344,13 +345,35 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
 #pragma unroll
       for (int sl = 0; sl < NSL; ++sl) soff[j][sl] *= 4u;
     }
-    row[j] = FILT ? (real*)a.filtration + sid[j] : nullptr;
-    if (FILT && live[j]) {
-#pragma unroll
-      for (int p = 0; p < P; ++p) __builtin_nontemporal_store(X[j][p], row[j] + (long long)p * S);
-    }
   }
-  const long long row_stride = (long long)P * S;
+  // Filtration rows are written through a buffer resource rebased on each row (scalar address
+  // arithmetic only): the path's byte offset is the 32-bit vector offset and the process's
+  // offset within the row the scalar offset; a row of P * S values must stay below 2 GiB. The
+  // stores carry the non-temporal cache policy (aux = 2): the filtration streams past L2, which
+  // keeps the Sobol tables resident (with the default policy the kernel runs ~1.5x slower).
+  const long long row_bytes = (long long)P * S * (long long)sizeof(real);
+  sde_u32 voff[PPL];
+#pragma unroll
+  for (int j = 0; j < PPL; ++j) voff[j] = (sde_u32)(sid[j] * (long long)sizeof(real));
+  auto store_row = [&](int r) {
+    char* base = (char*)a.filtration + (long long)r * row_bytes;
+    __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)row_bytes, 0x00020000);
+#pragma unroll
+    for (int j = 0; j < PPL; ++j) {
+      if (live[j]) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+          const int soff_p = (int)((long long)p * S * (long long)sizeof(real));
+          if constexpr (sizeof(real) == 4)
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(sde_u32, (float)X[j][p]), rs, (int)voff[j], soff_p, 2);
+          else
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(SdeU32x2, (double)X[j][p]), rs, (int)voff[j],
+                                                  soff_p, 2);
+        }
+      }
+    }
+  };
+  if (FILT) store_row(0);
 
   // Sobol table entries are read through a buffer resource: the uniform row offset goes in the
   // scalar offset and the path's fixed entry offsets in the 32-bit vector offset, so a gather
@@ -455,16 +478,7 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
         }
 #pragma unroll
         for (int j = 0; j < PPL; ++j) Mdl::step(X[j], C[j], K[j], gc.tc, gc.tn, gc.dt, gc.sdt, u[j]);
-        if (FILT) {
-#pragma unroll
-          for (int j = 0; j < PPL; ++j) {
-            row[j] += row_stride;
-            if (live[j]) {
-#pragma unroll
-              for (int p = 0; p < P; ++p) __builtin_nontemporal_store(X[j][p], row[j] + (long long)p * S);
-            }
-          }
-        }
+        if (FILT) store_row(n + 1);
         if (more) combine(n + 1);
       }
     }

@@ -10,8 +10,11 @@
 
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 #include <map>
+#include <sstream>
 #include <memory>
 #include <mutex>
 #include <set>
@@ -92,7 +95,18 @@ std::map<std::pair<int, std::string>, CompiledKernel> g_kernels;
 // Compile generated source to a gfx950 code object (needs no device).
 std::vector<char> rtc_compile(const std::string& src) {
   hiprtcProgram prog;
-  const char* hdrs[] = {sdesim_embedded_device_header, sdesim_embedded_args_header};
+  // Kernel development: SDESIM_DEVICE_HEADER names a device header file to compile run-time
+  // kernels against instead of the copy embedded at build time.
+  static const std::string dev_header = [] {
+    const char* path = std::getenv("SDESIM_DEVICE_HEADER");
+    if (!path || !*path) return std::string(sdesim_embedded_device_header);
+    std::ifstream in(path);
+    if (!in) return std::string(sdesim_embedded_device_header);
+    std::stringstream ss;
+    ss << in.rdbuf();
+    return ss.str();
+  }();
+  const char* hdrs[] = {dev_header.c_str(), sdesim_embedded_args_header};
   const char* names[] = {"sdesim_device.hpp", "sdesim_args.hpp"};
   if (hiprtcCreateProgram(&prog, src.c_str(), "sdesim_kernel.hip", 2, hdrs, names) != HIPRTC_SUCCESS)
     fail(SDESIM_ERUNTIME, "hiprtcCreateProgram failed");
@@ -452,6 +466,9 @@ int sdesim_plan_create(const sdesim_universe* uh, const double* times, int32_t T
         a.xor_mask = plan->own(upload(masks));
       }
     }
+    if ((opt.outputs & SDESIM_OUT_FILTRATION) &&
+        (long long)P * S * (opt.precision == SDESIM_F32 ? 4 : 8) >= (1LL << 31))
+      fail(SDESIM_EVALUE, "one filtration row (processes x scenarios) must be below 2 GiB; split the scenarios");
     plan->grid = (S + (long long)plan->block * plan->ppl - 1) / ((long long)plan->block * plan->ppl);
     if (plan->grid > 0x7fffffffLL) fail(SDESIM_EVALUE, "too many scenarios for one launch");
     if (opt.outputs & SDESIM_OUT_TERMINAL) {
