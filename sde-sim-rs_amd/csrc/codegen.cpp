@@ -341,7 +341,7 @@ std::string generate_kernel_source(const Universe& u, const CodegenOptions& o) {
   std::string attr = (wpe && std::atoi(wpe) > 0) ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(std::atoi(wpe)) + ")))" : "";
   s << "extern \"C\" __global__ void __launch_bounds__(" << o.block << ")" << attr << " " << o.name << "(SimArgs a) {\n";
   s << "  sde_path_kernel<SdeModel, " << o.rng << ", " << o.out << ", " << o.ppl << ", " << o.nsl << ", "
-    << (o.uniform_grid ? "true" : "false") << ">(a);\n";
+    << (o.uniform_grid ? "true" : "false") << ", " << (o.wave_aligned ? "true" : "false") << ">(a);\n";
   s << "}\n";
   return s.str();
 }

@@ -17,6 +17,7 @@ struct CodegenOptions {
   int block = 256;      // threads per block
   int nsl = 1;          // Sobol table slices (Gray-code bits covered: 6 + 8 (nsl - 1))
   bool uniform_grid = false;  // every dt and sqrt(dt) is bitwise equal in the kernel's precision
+  bool wave_aligned = false;  // Sobol index of path 0 is a multiple of 64 and blocks hold whole waves
   bool steps = true;    // false when the time grid has a single point (nothing is evaluated)
   std::string payoff;   // expression over process names and t at the last time point
   std::string name = "sdesim_kernel";  // entry point name

@@ -17,9 +17,9 @@ struct SimArgs {
   const float* dts_f;        // [N]
   const float* sdts_f;       // [N]
   const double* x0;          // [P]
-  const sde_u32* sobol_tab;  // [dims][64 + 256 * (nslices - 1)] sliced XOR tables
-  const sde_u32* xor_mask;   // [dims] digital-shift masks (XOR mode)
-  const double* injected;    // [N * M][S] injected uniforms (test hook)
+  const sde_u32* sobol_tab;  // [dims][64 + 256 * (nslices - 1)] sliced XOR tables; in XOR mode
+                             // slice 0 of dimension d already holds its digital-shift mask
+  const double* injected;   // [N * M][S] injected uniforms (test hook)
   void* filtration;          // [T][P][S] real, or null
   double* partials;          // [gridDim.x][2 * KQ] (sum, sum of squares) of shifted values, or null
   double dt_u, sdt_u;        // the step when every dt (and sqrt dt) is bitwise equal (f64 kernels)

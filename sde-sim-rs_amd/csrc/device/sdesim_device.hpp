@@ -301,7 +301,16 @@ __device__ __forceinline__ SdeGrid<R> sde_grid(const SimArgs& a, int n) {
   return g;
 }
 
-template <class Mdl, int RNG, int OUT, int PPL, int NSL, bool UG>
+#ifndef SDE_SOBOL_DEPTH
+#define SDE_SOBOL_DEPTH 1
+#endif
+
+typedef const __attribute__((address_space(4))) sde_u32 sde_const_u32;
+
+// AL: the Sobol index of path 0 is a multiple of 64 and a block holds whole waves, so the 64
+// paths of a wave are one aligned run of 64 consecutive point indices; their Gray codes agree
+// above bit 5 and every table slice but the first gives the same entry to the whole wave.
+template <class Mdl, int RNG, int OUT, int PPL, int NSL, bool UG, bool AL = false>
 __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
   typedef typename Mdl::real real;
   typedef typename SdeUnifOf<RNG>::type unif;
@@ -312,7 +321,10 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
   constexpr bool USE_CHACHA = (RNG == SDE_RNG_PSEUDO || RNG == SDE_RNG_SOBOL_SHIFT) && M > 0;
   constexpr bool USE_SOBOL = (RNG == SDE_RNG_SOBOL_SHIFT || RNG == SDE_RNG_SOBOL_XOR) && M > 0;
   constexpr bool USE_INJ = RNG == SDE_RNG_INJECTED && M > 0;
-  constexpr int U = USE_CHACHA ? SdeChunk<M>::U : 1;
+  // Table-driven uniforms are requested DEPTH steps ahead (DEPTH buffers, so the loop is
+  // unrolled by DEPTH and each buffer index is a constant).
+  constexpr int DEPTH = USE_SOBOL ? SDE_SOBOL_DEPTH : 1;
+  constexpr int U = USE_CHACHA ? SdeChunk<M>::U : DEPTH;
   constexpr int NB = SdeChunk<M>::BLOCKS;
   constexpr int KQ = P + (Mdl::HAS_PAYOFF ? 1 : 0);
   constexpr bool FILT = (OUT & SDE_OUT_FILTRATION) != 0;
@@ -325,6 +337,7 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
   real X[PPL][P], C[PPL][P], K[PPL][NK];
   sde_u32 key[PPL][8];
   sde_u32 soff[PPL][NSL];  // byte offsets of this path's entries within a dimension's table row
+  sde_u32 uoff[PPL][NSL];  // with AL: the wave's common entry index of slices 1.. (word units)
   long long sid[PPL];
   bool live[PPL];
 
@@ -343,7 +356,12 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
       const sde_u64 idx = (sde_u64)(a.sobol_first + sid[j]);
       sde_sobol_offsets<NSL>(idx ^ (idx >> 1), soff[j]);
 #pragma unroll
-      for (int sl = 0; sl < NSL; ++sl) soff[j][sl] *= 4u;
+      for (int sl = 0; sl < NSL; ++sl) {
+        // Lane 0 has the wave's smallest path index: if it is past the last path, so is every
+        // lane of the wave, whose values are then never used.
+        uoff[j][sl] = __builtin_amdgcn_readfirstlane(soff[j][sl]);
+        soff[j][sl] *= 4u;
+      }
     }
   }
   // Filtration rows are written through a buffer resource rebased on each row (scalar address
@@ -351,25 +369,27 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
   // offset within the row the scalar offset; a row of P * S values must stay below 2 GiB. The
   // stores carry the non-temporal cache policy (aux = 2): the filtration streams past L2, which
   // keeps the Sobol tables resident (with the default policy the kernel runs ~1.5x slower).
+  // Lanes past the last path store at vector offset 2^31, which is beyond the row resource's
+  // size whether or not the scalar offset is counted, so the hardware drops those stores; the
+  // stores are therefore unconditional and leave no exec-masked branch behind that would make
+  // the next wait for table loads also wait for this step's stores.
   const long long row_bytes = (long long)P * S * (long long)sizeof(real);
   sde_u32 voff[PPL];
 #pragma unroll
-  for (int j = 0; j < PPL; ++j) voff[j] = (sde_u32)(sid[j] * (long long)sizeof(real));
+  for (int j = 0; j < PPL; ++j) voff[j] = live[j] ? (sde_u32)(sid[j] * (long long)sizeof(real)) : 0x80000000u;
   auto store_row = [&](int r) {
     char* base = (char*)a.filtration + (long long)r * row_bytes;
     __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc((void*)base, (short)0, (int)row_bytes, 0x00020000);
 #pragma unroll
     for (int j = 0; j < PPL; ++j) {
-      if (live[j]) {
 #pragma unroll
-        for (int p = 0; p < P; ++p) {
-          const int soff_p = (int)((long long)p * S * (long long)sizeof(real));
-          if constexpr (sizeof(real) == 4)
-            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(sde_u32, (float)X[j][p]), rs, (int)voff[j], soff_p, 2);
-          else
-            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(SdeU32x2, (double)X[j][p]), rs, (int)voff[j],
-                                                  soff_p, 2);
-        }
+      for (int p = 0; p < P; ++p) {
+        const int soff_p = (int)((long long)p * S * (long long)sizeof(real));
+        if constexpr (sizeof(real) == 4)
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(sde_u32, (float)X[j][p]), rs, (int)voff[j], soff_p, 2);
+        else
+          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(SdeU32x2, (double)X[j][p]), rs, (int)voff[j],
+                                                soff_p, 2);
       }
     }
   };
@@ -380,14 +400,18 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
   // costs no address arithmetic (and reads outside the table would return 0, not fault).
   __amdgpu_buffer_rsrc_t tab_rsrc = __builtin_amdgcn_make_buffer_rsrc(
       (void*)a.sobol_tab, (short)0, USE_SOBOL ? a.sobol_tab_words * 4 : 0, 0x00020000);
+  // With AL the wave-uniform entries are read with scalar loads (through the constant address
+  // space; the index is below the table size because d < N * M and uoff < the row stride), which
+  // leaves one vector gather per increment and keeps these reads off the vector memory counter
+  // that the filtration stores also occupy.
+  sde_const_u32* tab_s = (sde_const_u32*)a.sobol_tab;
 
-  // Software pipeline for table-driven uniforms: the words of step n+1 are requested before
-  // step n is computed and combined into `nx` after it, so only one word per increment is
-  // carried from step to step.
-  sde_u32 sraw[PPL][MM][NSL];
+  // Software pipeline for table-driven uniforms: the words of step n+DEPTH are requested before
+  // step n is computed, and those of step n+1 are combined into `nx` after it.
+  sde_u32 sraw[DEPTH][PPL][MM][NSL];
   sde_u32 nx[PPL][MM];
   double ni[PPL][MM];
-  auto request = [&](int n) {
+  auto request = [&](int n, int b) {
 #pragma unroll
     for (int k = 0; k < MM; ++k) {
       const int d = n * M + k;
@@ -396,8 +420,12 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
 #pragma unroll
-          for (int sl = 0; sl < NSL; ++sl)
-            sraw[j][k][sl] = (sde_u32)__builtin_amdgcn_raw_buffer_load_b32(tab_rsrc, (int)soff[j][sl], base, 0);
+          for (int sl = 0; sl < NSL; ++sl) {
+            if (AL && sl > 0)
+              sraw[b][j][k][sl] = tab_s[(sde_u32)d * stride + uoff[j][sl]];
+            else
+              sraw[b][j][k][sl] = (sde_u32)__builtin_amdgcn_raw_buffer_load_b32(tab_rsrc, (int)soff[j][sl], base, 0);
+          }
         }
       }
       if (USE_INJ) {
@@ -406,23 +434,26 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
       }
     }
   };
-  auto combine = [&](int n) {
+  // XOR mode needs no mask here: the host folds dimension d's mask into the 64 slice-0 entries
+  // of its table row, so the XOR of the slices is already the scrambled point (and no per-step
+  // vector load of the mask forces a wait on this step's outstanding filtration stores).
+  auto combine = [&](int b) {
 #pragma unroll
     for (int k = 0; k < MM; ++k) {
       if (USE_SOBOL) {
-        const sde_u32 mask = RNG == SDE_RNG_SOBOL_XOR ? a.xor_mask[n * M + k] : 0u;
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
-          sde_u32 x = mask;
+          sde_u32 x = 0u;
 #pragma unroll
-          for (int sl = 0; sl < NSL; ++sl) x ^= sraw[j][k][sl];
+          for (int sl = 0; sl < NSL; ++sl) x ^= sraw[b][j][k][sl];
           nx[j][k] = x;
         }
       }
     }
   };
   if (N > 0) {
-    request(0);
+#pragma unroll
+    for (int b = 0; b < DEPTH; ++b) request(b < N ? b : N - 1, b);
     combine(0);
   }
   SdeGrid<real> g{};
@@ -470,16 +501,16 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
             }
           }
         }
+        // The grid and table words ahead are requested unconditionally (near the end the last
+        // step's are requested again, in bounds and unused), so no branch separates the loads
+        // from the stores below and the wait before `combine` covers the loads only.
         const SdeGrid<real> gc = g;
-        const bool more = n + 1 < N;
-        if (more) {
-          g = sde_grid<real, UG>(a, n + 1);
-          request(n + 1);
-        }
+        g = sde_grid<real, UG>(a, n + 1 < N ? n + 1 : n);
+        request(n + DEPTH < N ? n + DEPTH : N - 1, i % DEPTH);
 #pragma unroll
         for (int j = 0; j < PPL; ++j) Mdl::step(X[j], C[j], K[j], gc.tc, gc.tn, gc.dt, gc.sdt, u[j]);
         if (FILT) store_row(n + 1);
-        if (more) combine(n + 1);
+        combine((i + 1) % DEPTH);
       }
     }
   }

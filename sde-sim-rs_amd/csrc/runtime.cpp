@@ -416,6 +416,7 @@ int sdesim_plan_create(const sdesim_universe* uh, const double* times, int32_t T
     co.ppl = plan->ppl;
     co.block = plan->block;
     co.nsl = nslices;
+    co.wave_aligned = sobol_first % 64 == 0 && plan->block % 64 == 0;
     co.steps = N > 0;
     co.payoff = has_payoff ? payoff : "";
 
@@ -456,15 +457,22 @@ int sdesim_plan_create(const sdesim_universe* uh, const double* times, int32_t T
     a.T = T;
     a.nslices = nslices;
     a.sobol_first = sobol_first;
-    if (rng_kind == SDE_RNG_SOBOL_SHIFT || rng_kind == SDE_RNG_SOBOL_XOR) {
+    if (rng_kind == SDE_RNG_SOBOL_SHIFT) {
       SobolTables st = sobol_tables(std::max(1, N * M), nslices);
       a.sobol_tab = st.dev;
       a.sobol_tab_words = st.dims * (64 + 256 * (nslices - 1));
-      if (rng_kind == SDE_RNG_SOBOL_XOR) {
-        std::vector<unsigned int> masks;
-        chacha8_words(opt.scramble_seed, (size_t)std::max(1, N * M), masks);
-        a.xor_mask = plan->own(upload(masks));
-      }
+    } else if (rng_kind == SDE_RNG_SOBOL_XOR) {
+      // The plan's own tables with dimension d's mask XORed into its 64 slice-0 entries: every
+      // Gray code reads exactly one slice-0 entry, so the slices' XOR is point ^ mask.
+      const int dims = std::max(1, N * M);
+      const size_t stride = 64 + 256 * (size_t)(nslices - 1);
+      std::vector<unsigned int> tab, masks;
+      build_sobol_tables(dims, nslices, tab);
+      chacha8_words(opt.scramble_seed, (size_t)dims, masks);
+      for (int d = 0; d < dims; ++d)
+        for (int j = 0; j < 64; ++j) tab[(size_t)d * stride + j] ^= masks[d];
+      a.sobol_tab = plan->own(upload(tab));
+      a.sobol_tab_words = (int)(dims * stride);
     }
     if ((opt.outputs & SDESIM_OUT_FILTRATION) &&
         (long long)P * S * (opt.precision == SDESIM_F32 ? 4 : 8) >= (1LL << 31))
@@ -602,6 +610,8 @@ int sdesim_generate_source(const sdesim_universe* u, int32_t n_times, const sdes
     co.block = opt.block_size > 0 ? opt.block_size : 256;
     co.nsl = 3;
     co.uniform_grid = true;
+    const long long first = opt.sobol_first >= 0 ? opt.sobol_first : (co.rng == SDE_RNG_SOBOL_XOR ? 0 : 5);
+    co.wave_aligned = first % 64 == 0 && co.block % 64 == 0;
     co.name = sdesim::kernel_name(co);
     co.steps = n_times > 1;
     co.payoff = payoff ? payoff : "";

@@ -91,6 +91,22 @@ def test_heston_sobol(oracle, scramble, scheme):
     _close(f.transpose(2, 0, 1), ref)
 
 
+@pytest.mark.parametrize("scramble,first,ppl,block", [("xor", 37, 1, 256), ("xor", 192, 2, 128),
+                                                      ("xor", 0, 2, 64), ("shift", 0, 1, 64)])
+def test_sobol_first_index_and_launch_shape(oracle, scramble, first, ppl, block):
+    """A first point index that is a multiple of 64 lets a wave read its upper table slices with
+    scalar loads; any other index gathers every slice per lane. Both, over partial last waves
+    and several launch shapes, give the oracle's points."""
+    times = np.linspace(0.0, 1.0, 17)
+    S = 1000
+    print("case: heston sobol %s first=%d ppl=%d block=%d S=%d" % (scramble, first, ppl, block, S), flush=True)
+    _, f = _filtration(HESTON, times, S, {"S": 100.0, "v": 0.04}, "sobol", "runge-kutta", seed=99,
+                       scramble=scramble, scramble_seed=7, sobol_first=first, paths_per_lane=ppl, block_size=block)
+    ref = oracle.simulate(HESTON, times, S, {"S": 100.0, "v": 0.04}, scheme="runge-kutta", rng="sobol", seed=99,
+                          scramble=scramble, scramble_seed=7, sobol_first=first)
+    _close(f.transpose(2, 0, 1), ref)
+
+
 def test_terminal_moments_match_filtration():
     import sde_sim_rs
     times = np.linspace(0.0, 1.0, 101)
