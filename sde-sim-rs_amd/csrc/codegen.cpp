@@ -148,20 +148,21 @@ class StepEmitter {
     return name;
   }
 
-  std::string psi(int idx) {
-    std::string z = "z" + std::to_string(idx);
+  // The Wiener increment sqrt(dt) * Psi(u_idx), evaluated once per step.
+  std::string dw(int idx) {
+    std::string w = "w" + std::to_string(idx);
     if (!psi_.count(idx)) {
-      line("const " + R_ + " " + z + " = sde_psi<" + R_ + ">(u[" + std::to_string(idx) + "]);");
+      line("const " + R_ + " " + w + " = sde_dw(u[" + std::to_string(idx) + "], sdt);");
       psi_.insert(idx);
     }
-    return z;
+    return w;
   }
 
   // Incrementor::sample (src/proc/increment.rs:46-157).
   std::string sample(const Term& t) {
     switch (t.kind) {
       case IncKind::Time: return "dt";
-      case IncKind::Wiener: return "(sdt * " + psi(t.index) + ")";
+      case IncKind::Wiener: return dw(t.index);
       case IncKind::Jump: {
         std::string lam = eval(*t.lambda, false);
         std::string name = "j" + std::to_string(tmp_++);
@@ -190,6 +191,20 @@ class StepEmitter {
     for (int p : u_.levy) {
       const ProcessDef& pd = u_.processes[p];
       std::string acc = "a" + std::to_string(p);
+      if (o_.f32) {
+        // f32: the terms are summed first and the sum is added to the state strictly.
+        bool first = true;
+        for (const Term& t : pd.terms) {
+          std::string c = eval(*t.coef, false);
+          std::string x = sample(t);
+          if (first) line(R_ + " " + acc + " = " + c + " * " + x + ";");
+          else line(acc + " = " + acc + " + " + c + " * " + x + ";");
+          first = false;
+        }
+        if (first) line(y(p) + " = X[" + std::to_string(p) + "];");
+        else line(y(p) + " = sde_add_strict(X[" + std::to_string(p) + "], " + acc + ");");
+        continue;
+      }
       line(R_ + " " + acc + " = X[" + std::to_string(p) + "];");
       for (const Term& t : pd.terms) {
         std::string c = eval(*t.coef, false);
@@ -204,7 +219,6 @@ class StepEmitter {
   void rk() {
     if (u_.M() == 0) throw Panic("RNG Index 0 out of bounds (max 0)");
     if (o_.compat == 1) refresh(false);
-    line("const " + R_ + " sk = sde_gt_half(u[0]) ? " + R_ + "(1) : " + R_ + "(-1);");
     const int P = u_.P();
     std::vector<std::vector<std::string>> d(P);
     for (int p = 0; p < P; ++p) {
@@ -215,8 +229,8 @@ class StepEmitter {
     // Sums start at their first term rather than at 0 + first term (the reference's
     // `k1 = 0.0; k1 += c * d`): the values are identical except that an exact -0.0 term stays
     // -0.0, and the device can fuse every product into an FMA. (c * sk) * sdt is evaluated as
-    // c * (sk * sdt), exactly equal because sk = +-1.
-    line("const " + R_ + " sks = sk * sdt;");
+    // c * (sk * sdt), exactly equal because sk = +-1 (sk = 1 when u_0 > 1/2, else -1).
+    line("const " + R_ + " sks = sde_signed(u[0], sdt);");
     for (int p = 0; p < P; ++p) {
       const ProcessDef& pd = u_.processes[p];
       if (!pd.levy) continue;
@@ -254,9 +268,11 @@ class StepEmitter {
         else line(k2 + " = " + k2 + " + " + c + " * " + d[p][k] + ";");
       }
     }
-    for (int p : u_.levy)
-      line(y(p) + " = X[" + std::to_string(p) + "] + " + R_ + "(0.5) * (k1_" + std::to_string(p) + " + k2_" +
-           std::to_string(p) + ");");
+    for (int p : u_.levy) {
+      const std::string inc = R_ + "(0.5) * (k1_" + std::to_string(p) + " + k2_" + std::to_string(p) + ")";
+      if (o_.f32) line(y(p) + " = sde_add_strict(X[" + std::to_string(p) + "], " + inc + ");");
+      else line(y(p) + " = X[" + std::to_string(p) + "] + " + inc + ";");
+    }
     if (o_.compat == 1) refresh(true);
     algebraic_pass();
   }
@@ -316,6 +332,9 @@ std::string generate_kernel_source(const Universe& u, const CodegenOptions& o) {
   s << "  template <class UT>\n";
   s << "  __device__ static __forceinline__ void step(real (&X)[P], real (&C)[P], real (&K)[NK], real tc, real tn,\n";
   s << "                                              real dt, real sdt, const UT (&u)[" << (M > 0 ? M : 1) << "]) {\n";
+  // FP32 kernels let the compiler reassociate the step's arithmetic (for example 2 * (a - v)
+  // into one FMA); FP64 kernels keep the reference's evaluation order.
+  if (o.f32) s << "#pragma clang fp reassociate(on)\n";
   s << body << tail.str();
   s << "  }\n";
   s << "  __device__ static __forceinline__ real payoff(const real (&X)[P], real tt) {\n";

@@ -18,8 +18,9 @@ struct SimArgs {
   const float* sdts_f;       // [N]
   const double* x0;          // [P]
   const sde_u32* sobol_tab;  // [dims][64 + 256 * (nslices - 1)] sliced XOR tables; in XOR mode
-                             // slice 0 of dimension d already holds its digital-shift mask
-  const double* injected;   // [N * M][S] injected uniforms (test hook)
+                             // slice 0 of dimension d also holds the complement of its mask
+  const float* tail_tab;     // [1233] |z| / sqrt(2 ln 2) of XOR uniforms with qi < 1233 (|z| > 5)
+  const double* injected;    // [N * M][S] injected uniforms (test hook)
   void* filtration;          // [T][P][S] real, or null
   double* partials;          // [gridDim.x][2 * KQ] (sum, sum of squares) of shifted values, or null
   double dt_u, sdt_u;        // the step when every dt (and sqrt dt) is bitwise equal (f64 kernels)

@@ -101,11 +101,12 @@ __device__ __forceinline__ double sde_inv_normal(double p) {
 // f64 evaluation of |z| for a small q = min(p, 1 - p). The empty volatile asm makes the block
 // non-speculatable, so callers keep it behind a branch that waves skip when no lane needs it
 // (probability ~6e-7 per draw) instead of if-converting it into every step.
-__device__ __forceinline__ float sde_inv_normal_tail(double q) {
+// `scale` multiplies the result before it is rounded to f32.
+__device__ __forceinline__ float sde_inv_normal_tail(double q, double scale = 1.0) {
   asm volatile("");
   const double t = sqrt(-2.0 * log(q));
-  return (float)(t - ((0.010328 * t + 0.802853) * t + 2.515517) /
-                         (((0.001308 * t + 0.189269) * t + 1.432788) * t + 1.0));
+  return (float)(scale * (t - ((0.010328 * t + 0.802853) * t + 2.515517) /
+                                  (((0.001308 * t + 0.189269) * t + 1.432788) * t + 1.0)));
 }
 
 // The same rational approximation in f32. q = min(p, 1 - p) is formed in f64 (exact for a
@@ -126,22 +127,25 @@ __device__ __forceinline__ float sde_inv_normal_f32(double p) {
   return (p < 0.5) ? -x : x;
 }
 
-// A uniform from the digital-XOR-scrambled Sobol stream, kept as its 32-bit integer y; its
-// value is (y + 0.5) * 2^-32, strictly inside (0, 1).
+// A uniform from the digital-XOR-scrambled Sobol stream, kept as the complement yc = ~y of its
+// 32-bit integer y; its value is (y + 0.5) * 2^-32, strictly inside (0, 1). Bit 31 of yc is set
+// exactly when u < 1/2, which is the sign bit of Psi(u) and of sign(u - 1/2) sqrt(dt).
 struct SdeUnifX {
-  sde_u32 y;
+  sde_u32 yc;
+  const float* tail;  // the f32 increment's table of the 1233 tail values (|z| > 5), or null
 };
 
 __device__ __forceinline__ double sde_uval(double u) { return u; }
-__device__ __forceinline__ double sde_uval(SdeUnifX u) { return ((double)u.y + 0.5) * 0x1.0p-32; }
+__device__ __forceinline__ double sde_uval(SdeUnifX u) { return ((double)(~u.yc) + 0.5) * 0x1.0p-32; }
 __device__ __forceinline__ bool sde_gt_half(double u) { return u > 0.5; }
-__device__ __forceinline__ bool sde_gt_half(SdeUnifX u) { return u.y >= 0x80000000u; }
+__device__ __forceinline__ bool sde_gt_half(SdeUnifX u) { return (int)u.yc >= 0; }
 
-// f32 inverse normal of (y + 0.5) 2^-32 formed in the integer domain: q = min(u, 1 - u) is
-// (qi + 0.5) 2^-32 with qi = y or 2^32 - 1 - y. qi < 1233 (|z| > 5) takes the f64 path.
-__device__ __forceinline__ float sde_inv_normal_f32_x(sde_u32 y) {
-  const bool upper = y >= 0x80000000u;
-  const sde_u32 qi = upper ? ~y : y;
+// f32 inverse normal of (y + 0.5) 2^-32 formed in the integer domain from yc = ~y: q = min(u,
+// 1 - u) is (qi + 0.5) 2^-32 with qi = min(y, ~y) = yc ^ (yc >> 31, arithmetic). qi < 1233
+// (|z| > 5) takes the f64 path.
+__device__ __forceinline__ float sde_inv_normal_f32_x(sde_u32 yc) {
+  const bool upper = (int)yc >= 0;
+  const sde_u32 qi = yc ^ (sde_u32)((int)yc >> 31);
   float x;
   if (qi < 1233u) {
     x = sde_inv_normal_tail(((double)qi + 0.5) * 0x1.0p-32);
@@ -155,12 +159,62 @@ __device__ __forceinline__ float sde_inv_normal_f32_x(sde_u32 y) {
   return upper ? x : -x;
 }
 
+// Wiener increments sqrt(dt) * Psi(u) (src/proc/increment.rs:75-79) and the RK probe's
+// sign(u - 1/2) * sqrt(dt) (src/sim/runge_kutta.rs:22-27).
+//
+// f32 with XOR uniforms: with c = sqrt(2 ln 2) and t = c t', t' = sqrt(-log2 q) (the negation
+// rides on the square root's source modifier), the A&S value is z = c x' with
+//   x' = t' - num'(t') / den'(t'),  num'(t') = num(c t') / c,  den'(t') = den(c t'),
+// and c and the sign of z go into the multiplier: one bit-select on the sign mask that also
+// forms qi (sign of u - 1/2) picks +-c sqrt(dt).
+#define SDE_PSI_C 1.1774100225154747  // sqrt(2 ln 2)
+
+// |v| with the sign of u - 1/2 (bit 31 of yc): one v_bfi_b32 on the uniform's word. The sign
+// mask is materialised in a scalar register the compiler cannot see through (VOP3 takes no
+// literal on gfx9), so the select stays one instruction instead of an AND and an OR.
+__device__ __forceinline__ float sde_sign_select(sde_u32 yc, float v) {
+  sde_u32 m;
+  asm("s_mov_b32 %0, 0x80000000" : "=s"(m));
+  return __builtin_bit_cast(float, (m & yc) | (~m & __builtin_bit_cast(sde_u32, v)));
+}
+
+// The tail (qi < 1233, |z| > 5) takes only 1233 values, tabulated by the host in f64 with the
+// same formula and rounded to f32: tail[qi] = A&S((qi + 0.5) 2^-32) / c. The branch is rare
+// (~6e-7 per draw) and holds one load, so no f64 code (nor its hoisted constants) sits in the
+// step loop.
+__device__ __forceinline__ float sde_inv_normal_f32_x_scaled(sde_u32 yc, const float* tail) {
+  constexpr double c = SDE_PSI_C;
+  constexpr float A0 = (float)(2.515517 / c), A1 = 0.802853f, A2 = (float)(0.010328 * c);
+  constexpr float B1 = (float)(1.432788 * c), B2 = (float)(0.189269 * c * c), B3 = (float)(0.001308 * c * c * c);
+  const sde_u32 qi = yc ^ (sde_u32)((int)yc >> 31);
+  if (qi < 1233u) {
+    asm volatile("");
+    return tail[qi];
+  }
+  const float q = __builtin_fmaf((float)qi, 0x1.0p-32f, 0x1.0p-33f);
+  const float t = __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(q));
+  const float num = (A2 * t + A1) * t + A0;
+  const float den = ((B3 * t + B2) * t + B1) * t + 1.0f;
+  return t - num * __builtin_amdgcn_rcpf(den);
+}
+
 template <typename R> __device__ __forceinline__ R sde_psi(double p);
 template <> __device__ __forceinline__ double sde_psi<double>(double p) { return sde_inv_normal(p); }
 template <> __device__ __forceinline__ float sde_psi<float>(double p) { return sde_inv_normal_f32(p); }
 template <typename R> __device__ __forceinline__ R sde_psi(SdeUnifX u);
 template <> __device__ __forceinline__ double sde_psi<double>(SdeUnifX u) { return sde_inv_normal(sde_uval(u)); }
-template <> __device__ __forceinline__ float sde_psi<float>(SdeUnifX u) { return sde_inv_normal_f32_x(u.y); }
+template <> __device__ __forceinline__ float sde_psi<float>(SdeUnifX u) { return sde_inv_normal_f32_x(u.yc); }
+
+__device__ __forceinline__ double sde_dw(double u, double sdt) { return sdt * sde_inv_normal(u); }
+__device__ __forceinline__ float sde_dw(double u, float sdt) { return sdt * sde_inv_normal_f32(u); }
+__device__ __forceinline__ double sde_dw(SdeUnifX u, double sdt) { return sdt * sde_inv_normal(sde_uval(u)); }
+__device__ __forceinline__ float sde_dw(SdeUnifX u, float sdt) {
+  return sde_sign_select(u.yc, sdt * (float)SDE_PSI_C) * sde_inv_normal_f32_x_scaled(u.yc, u.tail);
+}
+
+template <typename R> __device__ __forceinline__ R sde_signed(double u, R sdt) { return (u > 0.5 ? R(1) : R(-1)) * sdt; }
+__device__ __forceinline__ double sde_signed(SdeUnifX u, double sdt) { return (sde_gt_half(u) ? 1.0 : -1.0) * sdt; }
+__device__ __forceinline__ float sde_signed(SdeUnifX u, float sdt) { return sde_sign_select(u.yc, sdt); }
 
 // Sequential inverse Poisson CDF capped at k = 200 (f64, as the reference).
 __device__ __forceinline__ double sde_inv_poisson(double u, double lambda) {
@@ -188,6 +242,10 @@ template <typename R> __device__ __forceinline__ R sde_or(R a, R b) { return sde
 template <typename R> __device__ __forceinline__ R sde_eq(R a, R b) { return sde_truthy(a - b) ? R(0) : R(1); }
 template <typename R> __device__ __forceinline__ R sde_ne(R a, R b) { return sde_truthy(a - b) ? R(1) : R(0); }
 template <typename R> __device__ __forceinline__ R sde_sq(R a) { return a * a; }
+// An addition the compiler may not reassociate (defined outside any reassociating scope): the
+// f32 step adds its increment to the state through it, so the state's rounding sequence does
+// not depend on how far the step loop is unrolled.
+template <typename R> __device__ __forceinline__ R sde_add_strict(R a, R b) { return a + b; }
 
 __device__ __forceinline__ double sde_sqrt(double a) { return sqrt(a); }
 __device__ __forceinline__ float sde_sqrt(float a) { return __builtin_amdgcn_sqrtf(a); }  // v_sqrt_f32, 1 ulp
@@ -434,9 +492,10 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
       }
     }
   };
-  // XOR mode needs no mask here: the host folds dimension d's mask into the 64 slice-0 entries
-  // of its table row, so the XOR of the slices is already the scrambled point (and no per-step
-  // vector load of the mask forces a wait on this step's outstanding filtration stores).
+  // XOR mode needs no mask here: the host folds the complement of dimension d's mask into the
+  // 64 slice-0 entries of its table row, so the XOR of the slices is already the complement of
+  // the scrambled point (and no per-step vector load of the mask forces a wait on this step's
+  // outstanding filtration stores).
   auto combine = [&](int b) {
 #pragma unroll
     for (int k = 0; k < MM; ++k) {
@@ -495,7 +554,8 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
               v = ni[j][k];
             }
             if constexpr (RNG == SDE_RNG_SOBOL_XOR) {
-              u[j][k].y = nx[j][k];
+              u[j][k].yc = nx[j][k];
+              u[j][k].tail = a.tail_tab;
             } else {
               u[j][k] = v;
             }

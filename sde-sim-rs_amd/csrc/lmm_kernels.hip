@@ -54,7 +54,8 @@ __device__ __forceinline__ void lmm_noise(const LmmArgs& a, __amdgpu_buffer_rsrc
 #pragma unroll
     for (int sl = 0; sl < NSL; ++sl) x ^= (sde_u32)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)soff[sl], base, 0);
     SdeUnifX u;
-    u.y = x;
+    u.yc = ~x;
+    u.tail = nullptr;
     eps[f] = sde_psi<R>(u);
   }
 }

@@ -273,6 +273,32 @@ int sdesim_set_error(int code, const std::string& msg) {
   return code;
 }
 
+// Tail values of the f32 XOR-mode increment: for qi = 0 .. 1232 (|z| > 5), the A&S inverse
+// normal (src/proc/increment.rs:160-179) of q = (qi + 0.5) 2^-32, divided by sqrt(2 ln 2) and
+// rounded to f32 (the kernel multiplies the signed sqrt(2 ln 2) sqrt(dt) back in). One copy per
+// device, kept until exit.
+std::mutex g_tail_mu;
+std::map<int, float*> g_tail;
+
+const float* tail_table() {
+  int dev = 0;
+  hip_check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lock(g_tail_mu);
+  float*& d = g_tail[dev];
+  if (d) return d;
+  std::vector<float> h(1233);
+  const double c = std::sqrt(2.0 * std::log(2.0));
+  for (int qi = 0; qi < 1233; ++qi) {
+    const double q = ((double)qi + 0.5) * 0x1.0p-32;
+    const double t = std::sqrt(-2.0 * std::log(q));
+    const double x = t - ((0.010328 * t + 0.802853) * t + 2.515517) / (((0.001308 * t + 0.189269) * t + 1.432788) * t + 1.0);
+    h[qi] = (float)(x / c);
+  }
+  hip_check(hipMalloc(&d, h.size() * sizeof(float)), "hipMalloc(tail table)");
+  hip_check(hipMemcpy(d, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice), "hipMemcpy(tail table)");
+  return d;
+}
+
 // Shared with the other runtime files: device Sobol tables and ChaCha8 words.
 const unsigned int* sdesim_device_sobol_tables(int dims, int nslices, int* words) {
   SobolTables t = sobol_tables(dims, nslices);
@@ -462,16 +488,18 @@ int sdesim_plan_create(const sdesim_universe* uh, const double* times, int32_t T
       a.sobol_tab = st.dev;
       a.sobol_tab_words = st.dims * (64 + 256 * (nslices - 1));
     } else if (rng_kind == SDE_RNG_SOBOL_XOR) {
-      // The plan's own tables with dimension d's mask XORed into its 64 slice-0 entries: every
-      // Gray code reads exactly one slice-0 entry, so the slices' XOR is point ^ mask.
+      // The plan's own tables with the complement of dimension d's mask XORed into its 64
+      // slice-0 entries: every Gray code reads exactly one slice-0 entry, so the slices' XOR is
+      // ~(point ^ mask), the form the kernel's uniforms take.
       const int dims = std::max(1, N * M);
       const size_t stride = 64 + 256 * (size_t)(nslices - 1);
       std::vector<unsigned int> tab, masks;
       build_sobol_tables(dims, nslices, tab);
       chacha8_words(opt.scramble_seed, (size_t)dims, masks);
       for (int d = 0; d < dims; ++d)
-        for (int j = 0; j < 64; ++j) tab[(size_t)d * stride + j] ^= masks[d];
+        for (int j = 0; j < 64; ++j) tab[(size_t)d * stride + j] ^= ~masks[d];
       a.sobol_tab = plan->own(upload(tab));
+      a.tail_tab = tail_table();
       a.sobol_tab_words = (int)(dims * stride);
     }
     if ((opt.outputs & SDESIM_OUT_FILTRATION) &&
