@@ -18,7 +18,9 @@ struct SimArgs {
   const float* sdts_f;       // [N]
   const double* x0;          // [P]
   const sde_u32* sobol_tab;  // [dims][64 + 256 * (nslices - 1)] sliced XOR tables; in XOR mode
-                             // slice 0 of dimension d also holds the complement of its mask
+                             // slice 0 of dimension d also holds the complement of its mask;
+                             // wave-aligned XOR mode: slice 0 only, as [N][64][M]
+  const sde_u32* sobol_up;   // wave-aligned XOR mode: slices 1.. as [nslices - 1][256][N * M]
   const float* tail_tab;     // [1233] |z| / sqrt(2 ln 2) of XOR uniforms with qi < 1233 (|z| > 5)
   const double* injected;    // [N * M][S] injected uniforms (test hook)
   void* filtration;          // [T][P][S] real, or null
@@ -32,6 +34,7 @@ struct SimArgs {
   int T;                     // time points
   int nslices;               // Sobol table slices
   int sobol_tab_words;       // entries in sobol_tab (bounds of its buffer resource)
+  int sobol_dims;            // wave-aligned XOR mode: N * M, the row length of sobol_up
 };
 
 #define SDE_RNG_PSEUDO 0

@@ -364,10 +364,33 @@ __device__ __forceinline__ SdeGrid<R> sde_grid(const SimArgs& a, int n) {
 #endif
 
 typedef const __attribute__((address_space(4))) sde_u32 sde_const_u32;
+typedef sde_u32 SdeU32x3 __attribute__((ext_vector_type(3)));
+typedef sde_u32 SdeU32x4 __attribute__((ext_vector_type(4)));
 
-// AL: the Sobol index of path 0 is a multiple of 64 and a block holds whole waves, so the 64
-// paths of a wave are one aligned run of 64 consecutive point indices; their Gray codes agree
-// above bit 5 and every table slice but the first gives the same entry to the whole wave.
+// W consecutive words at (voff + soff) of a buffer resource, in loads of up to 4 words.
+template <int W>
+__device__ __forceinline__ void sde_load_words(__amdgpu_buffer_rsrc_t r, int voff, int soff, sde_u32* out) {
+  if constexpr (W >= 4) {
+    const SdeU32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
+    sde_load_words<W - 4>(r, voff + 16, soff, out + 4);
+  } else if constexpr (W == 3) {
+    const SdeU32x3 v = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0);
+    out[0] = v.x; out[1] = v.y; out[2] = v.z;
+  } else if constexpr (W == 2) {
+    const SdeU32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+    out[0] = v.x; out[1] = v.y;
+  } else if constexpr (W == 1) {
+    out[0] = (sde_u32)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+  }
+}
+
+// AL (XOR mode): the Sobol index of path 0 is a multiple of 64 and a block holds whole waves,
+// so the 64 paths of a wave are one aligned run of 64 consecutive point indices; their Gray
+// codes agree above bit 5 and every table slice but the first gives the same entry to the
+// whole wave. The host then lays the tables out for that access: slice 0 as [N][64][M] (a
+// lane's M words of a step are one vector load) and slices 1.. as [slice][entry][N * M] (a
+// wave's words of a step are consecutive, read with scalar loads).
 template <class Mdl, int RNG, int OUT, int PPL, int NSL, bool UG, bool AL = false>
 __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
   typedef typename Mdl::real real;
@@ -395,7 +418,8 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
   real X[PPL][P], C[PPL][P], K[PPL][NK];
   sde_u32 key[PPL][8];
   sde_u32 soff[PPL][NSL];  // byte offsets of this path's entries within a dimension's table row
-  sde_u32 uoff[PPL][NSL];  // with AL: the wave's common entry index of slices 1.. (word units)
+                           // (with AL: soff[j][0] is the byte offset of its words in a step block)
+  sde_const_u32* urow[PPL][NSL];  // with AL: the wave's row of slice sl >= 1 in sobol_up
   long long sid[PPL];
   bool live[PPL];
 
@@ -412,13 +436,20 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
     if (USE_CHACHA) sde_seed_from_u64(a.seed + (sde_u64)sid[j], key[j]);
     if (USE_SOBOL) {
       const sde_u64 idx = (sde_u64)(a.sobol_first + sid[j]);
-      sde_sobol_offsets<NSL>(idx ^ (idx >> 1), soff[j]);
+      const sde_u64 g = idx ^ (idx >> 1);
+      if (AL) {
+        soff[j][0] = (sde_u32)(g & 63u) * (sde_u32)(MM * 4);
 #pragma unroll
-      for (int sl = 0; sl < NSL; ++sl) {
-        // Lane 0 has the wave's smallest path index: if it is past the last path, so is every
-        // lane of the wave, whose values are then never used.
-        uoff[j][sl] = __builtin_amdgcn_readfirstlane(soff[j][sl]);
-        soff[j][sl] *= 4u;
+        for (int sl = 1; sl < NSL; ++sl) {
+          // Lane 0 has the wave's smallest path index: if it is past the last path, so is every
+          // lane of the wave, whose values are then never used.
+          const sde_u32 e = __builtin_amdgcn_readfirstlane((sde_u32)(g >> (6 + 8 * (sl - 1))) & 255u);
+          urow[j][sl] = (sde_const_u32*)a.sobol_up + ((size_t)(sl - 1) * 256u + e) * (size_t)a.sobol_dims;
+        }
+      } else {
+        sde_sobol_offsets<NSL>(g, soff[j]);
+#pragma unroll
+        for (int sl = 0; sl < NSL; ++sl) soff[j][sl] *= 4u;
       }
     }
   }
@@ -458,11 +489,9 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
   // costs no address arithmetic (and reads outside the table would return 0, not fault).
   __amdgpu_buffer_rsrc_t tab_rsrc = __builtin_amdgcn_make_buffer_rsrc(
       (void*)a.sobol_tab, (short)0, USE_SOBOL ? a.sobol_tab_words * 4 : 0, 0x00020000);
-  // With AL the wave-uniform entries are read with scalar loads (through the constant address
-  // space; the index is below the table size because d < N * M and uoff < the row stride), which
-  // leaves one vector gather per increment and keeps these reads off the vector memory counter
-  // that the filtration stores also occupy.
-  sde_const_u32* tab_s = (sde_const_u32*)a.sobol_tab;
+  // With AL the wave-uniform words are read with scalar loads (through the constant address
+  // space; indices stay below N * M), which leaves one vector load per step and keeps these
+  // reads off the vector memory counter that the filtration stores also occupy.
 
   // Software pipeline for table-driven uniforms: the words of step n+DEPTH are requested before
   // step n is computed, and those of step n+1 are combined into `nx` after it.
@@ -470,6 +499,20 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
   sde_u32 nx[PPL][MM];
   double ni[PPL][MM];
   auto request = [&](int n, int b) {
+    if (USE_SOBOL && AL) {
+#pragma unroll
+      for (int j = 0; j < PPL; ++j) {
+        sde_u32 w[MM];
+        sde_load_words<MM>(tab_rsrc, (int)soff[j][0], n * (64 * MM * 4), w);
+#pragma unroll
+        for (int k = 0; k < MM; ++k) {
+          sraw[b][j][k][0] = w[k];
+#pragma unroll
+          for (int sl = 1; sl < NSL; ++sl) sraw[b][j][k][sl] = urow[j][sl][n * M + k];
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int k = 0; k < MM; ++k) {
       const int d = n * M + k;
@@ -478,12 +521,8 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
 #pragma unroll
         for (int j = 0; j < PPL; ++j) {
 #pragma unroll
-          for (int sl = 0; sl < NSL; ++sl) {
-            if (AL && sl > 0)
-              sraw[b][j][k][sl] = tab_s[(sde_u32)d * stride + uoff[j][sl]];
-            else
-              sraw[b][j][k][sl] = (sde_u32)__builtin_amdgcn_raw_buffer_load_b32(tab_rsrc, (int)soff[j][sl], base, 0);
-          }
+          for (int sl = 0; sl < NSL; ++sl)
+            sraw[b][j][k][sl] = (sde_u32)__builtin_amdgcn_raw_buffer_load_b32(tab_rsrc, (int)soff[j][sl], base, 0);
         }
       }
       if (USE_INJ) {

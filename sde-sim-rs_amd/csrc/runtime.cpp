@@ -442,7 +442,7 @@ int sdesim_plan_create(const sdesim_universe* uh, const double* times, int32_t T
     co.ppl = plan->ppl;
     co.block = plan->block;
     co.nsl = nslices;
-    co.wave_aligned = sobol_first % 64 == 0 && plan->block % 64 == 0;
+    co.wave_aligned = rng_kind == SDE_RNG_SOBOL_XOR && sobol_first % 64 == 0 && plan->block % 64 == 0;
     co.steps = N > 0;
     co.payoff = has_payoff ? payoff : "";
 
@@ -498,9 +498,25 @@ int sdesim_plan_create(const sdesim_universe* uh, const double* times, int32_t T
       chacha8_words(opt.scramble_seed, (size_t)dims, masks);
       for (int d = 0; d < dims; ++d)
         for (int j = 0; j < 64; ++j) tab[(size_t)d * stride + j] ^= ~masks[d];
-      a.sobol_tab = plan->own(upload(tab));
+      if (co.wave_aligned && N > 0) {
+        // Wave-aligned layout: slice 0 as [N][64][M], slices 1.. as [nslices - 1][256][N * M].
+        std::vector<unsigned int> t0((size_t)N * 64 * M), up((size_t)(nslices - 1) * 256 * dims);
+        for (int n = 0; n < N; ++n)
+          for (int e = 0; e < 64; ++e)
+            for (int k = 0; k < M; ++k) t0[((size_t)n * 64 + e) * M + k] = tab[(size_t)(n * M + k) * stride + e];
+        for (int sl = 1; sl < nslices; ++sl)
+          for (int e = 0; e < 256; ++e)
+            for (int d = 0; d < dims; ++d)
+              up[((size_t)(sl - 1) * 256 + e) * dims + d] = tab[(size_t)d * stride + 64 + 256 * (sl - 1) + e];
+        a.sobol_tab = plan->own(upload(t0));
+        a.sobol_tab_words = (int)t0.size();
+        a.sobol_up = plan->own(upload(up));
+        a.sobol_dims = dims;
+      } else {
+        a.sobol_tab = plan->own(upload(tab));
+        a.sobol_tab_words = (int)(dims * stride);
+      }
       a.tail_tab = tail_table();
-      a.sobol_tab_words = (int)(dims * stride);
     }
     if ((opt.outputs & SDESIM_OUT_FILTRATION) &&
         (long long)P * S * (opt.precision == SDESIM_F32 ? 4 : 8) >= (1LL << 31))
@@ -639,7 +655,7 @@ int sdesim_generate_source(const sdesim_universe* u, int32_t n_times, const sdes
     co.nsl = 3;
     co.uniform_grid = true;
     const long long first = opt.sobol_first >= 0 ? opt.sobol_first : (co.rng == SDE_RNG_SOBOL_XOR ? 0 : 5);
-    co.wave_aligned = first % 64 == 0 && co.block % 64 == 0;
+    co.wave_aligned = co.rng == SDE_RNG_SOBOL_XOR && first % 64 == 0 && co.block % 64 == 0;
     co.name = sdesim::kernel_name(co);
     co.steps = n_times > 1;
     co.payoff = payoff ? payoff : "";
