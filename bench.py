@@ -56,6 +56,24 @@ def _time_plan(plan, out, steps, warmup):
     return wall, kern
 
 
+def _time_fill(buf, steps, warmup):
+    """Mean ms of a plain fill of the same buffer (torch's elementwise fill kernel): the write
+    rate this many bytes reach on this GPU without any computation in the loop."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    for _ in range(max(1, warmup)):
+        buf.fill_(1.0)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    for _ in range(steps):
+        buf.fill_(1.0)
+    b.record(stream)
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / steps
+
+
 def _profiled_traffic(paths, nsteps):
     """HBM bytes per launch of the headline kernel from the committed rocprofv3 counter summary
     (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy); None for other sizes."""
@@ -95,7 +113,7 @@ def main():
     ap.add_argument("--nsteps", type=int, default=1024)
     ap.add_argument("--ppl", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--cpu-paths", type=int, default=1 << 17)
+    ap.add_argument("--cpu-paths", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true", help="run a 1 GiB device copy first (HBM counter calibration)")
     ap.add_argument("--only", default="", help="comma list of legs to run: f32,f64,term,batch,adjoint,lmm")
@@ -127,6 +145,7 @@ def main():
         p32 = sde_sim_rs.Plan(uni, times, S, INIT, "sobol", "runge-kutta", precision="f32", **common)
         f32 = p32.new_filtration()
         res["f32"] = _time_plan(p32, f32, args.steps, args.warmup)
+        res["fill_ms"] = _time_fill(f32, args.steps, args.warmup)
         del f32
     if "f64" in legs:
         p64 = sde_sim_rs.Plan(uni, times, S, INIT, "sobol", "runge-kutta", precision="f64", **common)
@@ -229,6 +248,8 @@ def main():
                             "frac": ach / HBM_PEAK_GBS, "traffic": _profiled_traffic(S, N),
                             "kernel": "sdesim_rk_f32_sobolxor_filt", "kernel_ms": kern,
                             "algorithmic_bytes": bytes_alg}
+        fill_gbs = bytes_alg / (res["fill_ms"] * 1e-3) / 1e9
+        line["roofline"].update({"fill_same_bytes_gbs": fill_gbs, "frac_of_fill": ach / fill_gbs})
     if "f64" in res:
         wall, kern = res["f64"]
         bytes_alg = (N + 1) * 2 * S * 8
