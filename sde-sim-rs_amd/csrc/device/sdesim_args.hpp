@@ -89,13 +89,15 @@ struct BasketArgs {
 // Forward-rate model, price and curve sensitivities (lmm_kernels.hip); 40 forwards, 3 factors
 // ------------------------------------------------------------------------------------------
 struct LmmArgs {
-  double L0[40], vol[40], load[120];  // initial forwards, volatilities, loadings [40][3]
-  float volf[40], loadf[120];         // f32 copies for f32 kernels
+  double L0[40];                      // initial forwards
+  const double* pard;                 // device [40] volatilities then [40][3] loadings (f64)
+  const float* parf;                  // the same in f32
   double dt, sdt, tau, strike;
   long long paths;
   const sde_u32* sobol_tab;           // sliced Sobol tables (3 slices)
   const sde_u32* xor_mask;            // [steps * 3]
-  void* checkpoints;                  // [slots][40][paths] real
+  void* checkpoints;                  // [slots][40][paths] real, one slot per 16 steps
+  void* tape;                         // [16][40][paths] real, the states of one segment
   double* partials;                   // [gridDim.x][42]
   int steps;
   int steps_per_period;

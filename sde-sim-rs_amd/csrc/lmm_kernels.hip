@@ -14,9 +14,11 @@
 // fixed rate K: V = max(K A - (1 - P_{NF-1}), 0) / N, P_j = prod_{k=e..j} 1 / (1 + tau L_k),
 // A = tau sum_{j=e..NF-1} P_j, numeraire N = prod_{k<e} (1 + tau L_k).
 //
-// Adjoint: the forward pass stores L every SEG steps (checkpoints in HBM, [slot][i][path]); the
-// reverse sweep recomputes each segment from its checkpoint and steps back through it. For one
-// step, with a_j = tau sig_j L_j / (1 + tau L_j), E_i = L'_i / L_i:
+// Adjoint: the forward pass stores L every SEG steps (checkpoints in HBM, [slot][i][path]). The
+// reverse sweep takes the segments last to first: it recomputes a segment's states once from its
+// checkpoint into the segment tape ([k][i][path] in HBM, coalesced, rewritten per segment) and
+// then steps back through them. For one step, with a_j = tau sig_j L_j / (1 + tau L_j) and
+// E_i = L'_i / L_i (read as the ratio of the stored states, so the reverse step needs no exp):
 //   Cbar_f(i) = Lbar'_i L'_i dt sig_i B_if,   G_f(j) = sum_{i >= j} Cbar_f(i)  (a suffix sum),
 //   Lbar_j = Lbar'_j E_j + (sum_f B_jf G_f(j)) tau sig_j / (1 + tau L_j)^2.
 #include <hip/hip_runtime.h>
@@ -27,7 +29,7 @@ namespace {
 
 constexpr int NF = 40;
 constexpr int F = 3;
-constexpr int SEG = 8;
+constexpr int SEG = 16;  // checkpoint interval (~sqrt of the 160-step workload)
 constexpr int NSL = 3;  // Sobol table slices: point indices below 2^22
 
 template <typename R>
@@ -60,22 +62,20 @@ __device__ __forceinline__ void lmm_noise(const LmmArgs& a, __amdgpu_buffer_rsrc
   }
 }
 
-// Model parameters live in the kernel-argument block. Each step reads them at index i + off,
-// where off = step * a.zero is 0 at run time but not known to the compiler, so the reads are
-// uniform scalar loads repeated per step instead of 160 loop-invariant values hoisted into
-// vector registers for the whole kernel.
+// Model parameters live in a device buffer ([40] volatilities then [40][3] loadings, in the
+// kernel's precision) read through the constant address space. Each step reads them at index
+// i + off, where off = step * a.zero is 0 at run time but not known to the compiler, so the reads
+// are uniform scalar loads repeated per step instead of 160 loop-invariant values hoisted into
+// registers for the whole kernel.
 template <typename R>
 struct LmmPar {
   const LmmArgs& a;
   int off;
-  __device__ __forceinline__ R vol(int i) const {
-    if constexpr (sizeof(R) == 4) return a.volf[i + off];
-    else return a.vol[i + off];
+  __device__ __forceinline__ const __attribute__((address_space(4))) R* par() const {
+    return (const __attribute__((address_space(4))) R*)(sizeof(R) == 4 ? (const void*)a.parf : (const void*)a.pard);
   }
-  __device__ __forceinline__ R load(int i, int f) const {
-    if constexpr (sizeof(R) == 4) return a.loadf[(i + off) * F + f];
-    else return a.load[(i + off) * F + f];
-  }
+  __device__ __forceinline__ R vol(int i) const { return par()[i + off]; }
+  __device__ __forceinline__ R load(int i, int f) const { return par()[NF + (i + off) * F + f]; }
 };
 
 // One log-Euler step of all alive forwards (forwards below `eta` are dead and unchanged).
@@ -99,30 +99,17 @@ __device__ __forceinline__ void lmm_step(const LmmPar<R>& tab, R (&L)[NF], const
   }
 }
 
-// Adjoint of lmm_step: on entry L is the state before the step and Lb the adjoint of the state
-// after it; on exit Lb is the adjoint of the state before it.
 template <typename R>
-__device__ __forceinline__ void lmm_step_adjoint(const LmmPar<R>& tab, const R (&L)[NF], R (&Lb)[NF],
-                                                 const R (&eps)[F], int eta, R dt, R sdt, R tau) {
-  R E[NF], Lp[NF];
-  R C[F] = {R(0), R(0), R(0)};
-#pragma unroll
-  for (int i = 0; i < NF; ++i) {
-    E[i] = R(1);
-    Lp[i] = L[i];
-    if (i >= eta) {
-      const R sig = tab.vol(i);
-      const R b0 = tab.load(i, 0), b1 = tab.load(i, 1), b2 = tab.load(i, 2);
-      const R ai = tau * sig * L[i] / (R(1) + tau * L[i]);
-      C[0] = C[0] + ai * b0;
-      C[1] = C[1] + ai * b1;
-      C[2] = C[2] + ai * b2;
-      const R drift = sig * (b0 * C[0] + b1 * C[1] + b2 * C[2]) - R(0.5) * sig * sig;
-      const R w = b0 * eps[0] + b1 * eps[1] + b2 * eps[2];
-      E[i] = lmm_exp<R>(drift * dt + sig * sdt * w);
-      Lp[i] = L[i] * E[i];
-    }
-  }
+__device__ __forceinline__ R lmm_recip(R x) {
+  if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rcpf(x);  // v_rcp_f32, 1 ulp
+  else return R(1) / x;
+}
+
+// Adjoint of lmm_step: L is the state before the step and Lp the state after it; on entry Lb is
+// the adjoint of Lp, on exit the adjoint of L. Forwards are positive, so E_i = Lp_i / L_i.
+template <typename R>
+__device__ __forceinline__ void lmm_step_adjoint(const LmmPar<R>& tab, const R (&L)[NF], const R (&Lp)[NF],
+                                                 R (&Lb)[NF], int eta, R dt, R tau) {
   R G[F] = {R(0), R(0), R(0)};
 #pragma unroll
   for (int i = NF - 1; i >= 0; --i) {
@@ -134,8 +121,8 @@ __device__ __forceinline__ void lmm_step_adjoint(const LmmPar<R>& tab, const R (
       G[1] = G[1] + db * sig * b1;
       G[2] = G[2] + db * sig * b2;
       const R ab = b0 * G[0] + b1 * G[1] + b2 * G[2];
-      const R op = R(1) + tau * L[i];
-      Lb[i] = Lb[i] * E[i] + ab * tau * sig / (op * op);
+      const R rop = lmm_recip<R>(R(1) + tau * L[i]);
+      Lb[i] = Lb[i] * (Lp[i] * lmm_recip<R>(L[i])) + ab * tau * sig * (rop * rop);
     }
   }
 }
@@ -216,33 +203,36 @@ __global__ void __launch_bounds__(256) sdesim_lmm(const LmmArgs a) {
         Lb[k] = U > R(0) ? (-K * tau * tau * tail - tau * P) / (op * numer) : R(0);
       }
     }
+    // Lp holds the state after the step being reversed: L_N first, then each state read back.
+    R Lp[NF];
+#pragma unroll
+    for (int i = 0; i < NF; ++i) Lp[i] = L[i];
+    R* tape = (R*)a.tape;
     const int nseg = (N + SEG - 1) / SEG;
     for (int seg = nseg - 1; seg >= 0; --seg) {
       const int n0 = seg * SEG;
-      R Xs[SEG][NF];
+      const int len = N - n0 < SEG ? N - n0 : SEG;
       R Xc[NF];
 #pragma unroll
       for (int i = 0; i < NF; ++i) Xc[i] = ck[((long long)seg * NF + i) * S + sid];
-      for (int k = 0; k < SEG; ++k) {
+      // States before steps n0 .. n0 + len - 2 go to the tape; the last one stays in Xc.
+      for (int k = 0; k + 1 < len; ++k) {
+        if (live) {
 #pragma unroll
-        for (int i = 0; i < NF; ++i) Xs[k][i] = Xc[i];
-        if (n0 + k < N && k + 1 < SEG) {
-          R eps[F];
-          lmm_noise<R>(a, rsrc, soff, n0 + k, eps);
-          lmm_step<R>(LmmPar<R>{a, (n0 + k) * a.zero}, Xc, eps, (n0 + k) / a.steps_per_period + 1, dt, sdt,
-                      tau);
+          for (int i = 0; i < NF; ++i) tape[((long long)k * NF + i) * S + sid] = Xc[i];
         }
+        R eps[F];
+        lmm_noise<R>(a, rsrc, soff, n0 + k, eps);
+        lmm_step<R>(LmmPar<R>{a, (n0 + k) * a.zero}, Xc, eps, (n0 + k) / a.steps_per_period + 1, dt, sdt, tau);
       }
-      for (int k = SEG - 1; k >= 0; --k) {
-        if (n0 + k < N) {
-          R eps[F];
-          lmm_noise<R>(a, rsrc, soff, n0 + k, eps);
-          R Lk[NF];
+      for (int k = len - 1; k >= 0; --k) {
+        if (k + 1 < len) {
 #pragma unroll
-          for (int i = 0; i < NF; ++i) Lk[i] = Xs[k][i];
-          lmm_step_adjoint<R>(LmmPar<R>{a, (n0 + k) * a.zero}, Lk, Lb, eps, (n0 + k) / a.steps_per_period + 1, dt,
-                              sdt, tau);
+          for (int i = 0; i < NF; ++i) Xc[i] = tape[((long long)k * NF + i) * S + sid];
         }
+        lmm_step_adjoint<R>(LmmPar<R>{a, (n0 + k) * a.zero}, Xc, Lp, Lb, (n0 + k) / a.steps_per_period + 1, dt, tau);
+#pragma unroll
+        for (int i = 0; i < NF; ++i) Lp[i] = Xc[i];
       }
     }
     if (live) {

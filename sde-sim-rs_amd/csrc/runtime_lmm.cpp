@@ -18,7 +18,7 @@ const unsigned int* sdesim_device_sobol_tables(int dims, int nslices, int* words
 void sdesim_chacha8_words(unsigned long long seed, size_t n, std::vector<unsigned int>& out);
 
 namespace {
-constexpr int kNF = 40, kF = 3, kSeg = 8, kNQ = 2 + kNF;
+constexpr int kNF = 40, kF = 3, kSeg = 16, kNQ = 2 + kNF;
 
 struct LFail {
   int code;
@@ -47,10 +47,13 @@ struct sdesim_lmm {
   int tab_words = 0;
   unsigned int* masks = nullptr;
   void* ckpt = nullptr;
+  void* tape = nullptr;
+  double* pard = nullptr;
+  float* parf = nullptr;
   double* partials = nullptr;
   double* sums = nullptr;
   ~sdesim_lmm() {
-    for (void* p : {(void*)masks, ckpt, (void*)partials, (void*)sums})
+    for (void* p : {(void*)masks, ckpt, tape, (void*)pard, (void*)parf, (void*)partials, (void*)sums})
       if (p) (void)hipFree(p);
   }
 };
@@ -80,6 +83,9 @@ int sdesim_lmm_create(const sdesim_lmm_spec* spec, sdesim_lmm** out) {
     const size_t rb = s.precision == SDESIM_F32 ? 4 : 8;
     const size_t slots = (size_t)(s.steps + kSeg - 1) / kSeg;
     lk(hipMalloc(&m->ckpt, slots * kNF * (size_t)s.paths * rb), "hipMalloc(checkpoints)");
+    lk(hipMalloc(&m->tape, (size_t)kSeg * kNF * (size_t)s.paths * rb), "hipMalloc(segment tape)");
+    lk(hipMalloc(&m->pard, sizeof(double) * kNF * (1 + kF)), "hipMalloc(parameters)");
+    lk(hipMalloc(&m->parf, sizeof(float) * kNF * (1 + kF)), "hipMalloc(parameters)");
     lk(hipMalloc(&m->partials, sizeof(double) * kNQ * m->grid), "hipMalloc(partials)");
     lk(hipMalloc(&m->sums, sizeof(double) * kNQ), "hipMalloc(sums)");
     *out = m.release();
@@ -95,15 +101,21 @@ int sdesim_lmm_eval(sdesim_lmm* m, const double* L0, const double* vol, const do
     const sdesim_lmm_spec& s = m->spec;
     LmmArgs a;
     std::memset(&a, 0, sizeof a);
+    std::vector<double> pd(kNF * (1 + kF));
+    std::vector<float> pf(kNF * (1 + kF));
     for (int i = 0; i < kNF; ++i) {
       a.L0[i] = L0[i];
-      a.vol[i] = vol[i];
-      a.volf[i] = (float)vol[i];
-      for (int f = 0; f < kF; ++f) {
-        a.load[i * kF + f] = loadings[i * kF + f];
-        a.loadf[i * kF + f] = (float)loadings[i * kF + f];
-      }
+      pd[i] = vol[i];
+      for (int f = 0; f < kF; ++f) pd[kNF + i * kF + f] = loadings[i * kF + f];
     }
+    for (size_t q = 0; q < pd.size(); ++q) pf[q] = (float)pd[q];
+    hipStream_t st = (hipStream_t)stream;
+    // Ordered before the launch on the same stream; the host vectors outlive the synchronous
+    // wait at the end of this call.
+    lk(hipMemcpyAsync(m->pard, pd.data(), sizeof(double) * pd.size(), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    lk(hipMemcpyAsync(m->parf, pf.data(), sizeof(float) * pf.size(), hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+    a.pard = m->pard;
+    a.parf = m->parf;
     const double T = s.tenor * s.expiry_index;
     a.dt = T / s.steps;
     a.sdt = std::sqrt(a.dt);
@@ -114,11 +126,11 @@ int sdesim_lmm_eval(sdesim_lmm* m, const double* L0, const double* vol, const do
     a.sobol_tab_words = m->tab_words;
     a.xor_mask = m->masks;
     a.checkpoints = m->ckpt;
+    a.tape = m->tape;
     a.partials = m->partials;
     a.steps = s.steps;
     a.steps_per_period = s.steps_per_period;
     a.expiry_index = s.expiry_index;
-    hipStream_t st = (hipStream_t)stream;
     lk(sdesim_launch_lmm(s.precision == SDESIM_F32 ? 1 : 0, adjoint ? 1 : 0, (int)m->grid, a, st), "lmm launch");
     lk(sdesim_launch_sum_partials(m->partials, (int)m->grid, kNQ, m->sums, st), "lmm reduce");
     std::vector<double> sums(kNQ);
