@@ -107,6 +107,48 @@ def test_sobol_first_index_and_launch_shape(oracle, scramble, first, ppl, block)
     _close(f.transpose(2, 0, 1), ref)
 
 
+@pytest.mark.parametrize("m", [1, 3, 5])
+@pytest.mark.parametrize("scheme", ["euler", "runge-kutta"])
+def test_xor_increment_counts(oracle, m, scheme):
+    """XOR mode with 1, 3 and 5 increments per step: the wave-aligned table form loads a lane's m
+    slice-0 words of a step in one to two vector loads (one, three, four + one words)."""
+    eqs = ["dX = (0.05 * X) * dt + " + " + ".join("(%r * X) * dW%d" % (0.1 + 0.02 * k, k + 1) for k in range(m))]
+    times = np.linspace(0.0, 1.0, 9)
+    S = 1000
+    print("case: xor m=%d %s S=%d" % (m, scheme, S), flush=True)
+    _, f = _filtration(eqs, times, S, {"X": 100.0}, "sobol", scheme, seed=5, scramble="xor", scramble_seed=11)
+    ref = oracle.simulate(eqs, times, S, {"X": 100.0}, scheme=scheme, rng="sobol", seed=5, scramble="xor",
+                          scramble_seed=11)
+    _close(f.transpose(2, 0, 1), ref)
+
+
+def test_fp32_xor_tail_increments(oracle):
+    """FP32 XOR increments with qi = min(y, ~y) < 1233 (|z| > 5) read the host-tabulated tail. Over
+    2^21 points of dimension 0 (van der Corput) some scrambled points land there; after one GBM
+    step every path's FP32 value matches FP64, the tail paths included."""
+    import sde_sim_rs
+    S = 1 << 21
+    eqs = ["dX = (0.05 * X) * dt + (0.2 * X) * dW1"]
+    times = np.array([0.0, 1.0])
+    y = oracle.sobol_points(1, S)[:, 0] ^ oracle.xor_masks(7, 1)[0]
+    qi = np.minimum(y, ~y)
+    tail = np.nonzero(qi < 1233)[0]
+    print("case: fp32 xor tail S=%d tail paths %d" % (S, tail.size), flush=True)
+    assert tail.size > 0
+    out = {}
+    for prec in ("f32", "f64"):
+        plan = sde_sim_rs.Plan(eqs, times, S, {"X": 100.0}, "sobol", "euler", precision=prec, scramble="xor",
+                               scramble_seed=7, seed=0)
+        f = plan.new_filtration()
+        plan.run(f)
+        torch.cuda.synchronize()
+        out[prec] = f[1, 0].double().cpu().numpy()
+    z = (out["f64"][tail] - 105.0) / 20.0
+    assert np.all(np.abs(z) > 5.0), z
+    np.testing.assert_allclose(out["f32"][tail], out["f64"][tail], rtol=0, atol=2e-4)
+    np.testing.assert_allclose(out["f32"], out["f64"], rtol=0, atol=2e-4)
+
+
 def test_terminal_moments_match_filtration():
     import sde_sim_rs
     times = np.linspace(0.0, 1.0, 101)
