@@ -11,11 +11,14 @@
 // Adjoint (no reference counterpart): pathwise derivatives of the payoff with respect to X0, mu,
 // sigma and L, per path, reduced over paths in f64. The forward pass stores the state every SEG
 // steps (checkpoints in HBM, [slot][asset][path]); the reverse sweep walks the segments
-// backwards, recomputes each segment's states from its checkpoint into registers and then steps
-// back through it, regenerating eps_n from the counter-based stream:
-//   Xbar_n = Xbar_{n+1} g_n,  g_n = 1 + mu dt + sigma sqrt(dt) Z_n
-//   mubar += Xbar_{n+1} X_n dt,  sigmabar += Xbar_{n+1} X_n sqrt(dt) Z_n
-//   Zbar_n = Xbar_{n+1} X_n sigma sqrt(dt),  Lbar_ik += Zbar_{n,i} eps_{n,k}  (k <= i)
+// backwards, recomputes each segment's states from its checkpoint and then steps back through
+// it. A reverse step reads the step's noise off the two states instead of regenerating it:
+//   g_n = X_{n+1} / X_n = 1 + mu dt + sigma sqrt(dt) Z_n,  d_n = (X_{n+1} - X_n) - X_n mu dt
+// (X_{n+1} - X_n is exact, the two being within a factor 2; forming 1 + mu dt first would round
+// mu dt ~ 2e-4 to the spacing of 1 and bias every recovered Z_n)
+//   Xbar_n = Xbar_{n+1} g_n,  mubar += Xbar_{n+1} X_n dt,  sigmabar += Xbar_{n+1} d_n / sigma
+//   Zbar_n = Xbar_{n+1} X_n sigma sqrt(dt),  Z_n = d_n / (X_n sigma sqrt(dt)),
+//   eps_n = L^-1 Z_n (forward substitution),  Lbar_ik += Zbar_{n,i} eps_{n,k}  (k <= i)
 #include <hip/hip_runtime.h>
 
 #include "device/sdesim_device.hpp"
@@ -73,6 +76,12 @@ __device__ __forceinline__ void basket_noise(const sde_u32 (&key)[8], int n, con
     for (int k = 0; k <= i; ++k) z = z + L[i][k] * st.eps[k];
     st.Z[i] = z;
   }
+}
+
+template <typename R>
+__device__ __forceinline__ R basket_recip(R x) {
+  if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rcpf(x);  // v_rcp_f32, 1 ulp
+  else return R(1) / x;
 }
 
 template <typename R>
@@ -149,6 +158,17 @@ __global__ void __launch_bounds__(256) sdesim_basket(const BasketArgs a) {
     for (int i = 0; i < D; ++i) { Xb[i] = dpay * w[i]; gm[i] = R(0); gs[i] = R(0); }
 #pragma unroll
     for (int q = 0; q < NL; ++q) gL[q] = R(0);
+    // Per-asset constants of the reverse step: mu dt, 1 / (sigma sqrt(dt)), 1 / L_ii.
+    R mdt[D], rsd[D], rLd[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      mdt[i] = mu[i] * dt;
+      rsd[i] = R(1) / (sig[i] * sdt);
+      rLd[i] = R(1) / L[i][i];
+    }
+    R Xp[D];  // the state after the step being reversed: X_N first, then each state stepped back to
+#pragma unroll
+    for (int i = 0; i < D; ++i) Xp[i] = X[i];
     const int nseg = (N + SEG - 1) / SEG;
     for (int seg = nseg - 1; seg >= 0; --seg) {
       const int n0 = seg * SEG;
@@ -172,29 +192,39 @@ __global__ void __launch_bounds__(256) sdesim_basket(const BasketArgs a) {
 #pragma unroll
       for (int k = SEG - 1; k >= 0; --k) {
         if (n0 + k < N) {
-          BasketStep<R, D> st;
-          basket_noise<R, D>(key, n0 + k, L, st);
-          R Zb[D];
+          R Zb[D], Z[D], eps[D];
 #pragma unroll
           for (int i = 0; i < D; ++i) {
-            const R xb = Xb[i], x = Xs[k][i];
-            const R g = R(1) + mu[i] * dt + sig[i] * sdt * st.Z[i];
+            const R xb = Xb[i], x = Xs[k][i], xp = Xp[i];
+            const R rx = basket_recip<R>(x);
+            const R d = (xp - x) - x * mdt[i];
             gm[i] = gm[i] + xb * x * dt;
-            gs[i] = gs[i] + xb * x * sdt * st.Z[i];
+            gs[i] = gs[i] + xb * d;
             Zb[i] = xb * x * sig[i] * sdt;
-            Xb[i] = xb * g;
+            Z[i] = d * rx * rsd[i];
+            Xb[i] = xb * (xp * rx);
+            Xp[i] = x;
+          }
+#pragma unroll
+          for (int i = 0; i < D; ++i) {
+            R r = Z[i];
+#pragma unroll
+            for (int kk = 0; kk < i; ++kk) r = r - L[i][kk] * eps[kk];
+            eps[i] = r * rLd[i];
           }
 #pragma unroll
           for (int i = 0; i < D; ++i) {
 #pragma unroll
             for (int kk = 0; kk <= i; ++kk) {
               const int q = i * (i + 1) / 2 + kk;
-              gL[q] = gL[q] + Zb[i] * st.eps[kk];
+              gL[q] = gL[q] + Zb[i] * eps[kk];
             }
           }
         }
       }
     }
+#pragma unroll
+    for (int i = 0; i < D; ++i) gs[i] = gs[i] / sig[i];
     if (live) {
 #pragma unroll
       for (int i = 0; i < D; ++i) {
