@@ -100,9 +100,11 @@ __device__ __forceinline__ double sde_inv_normal(double p) {
 
 // f64 evaluation of |z| for a small q = min(p, 1 - p). The empty volatile asm makes the block
 // non-speculatable, so callers keep it behind a branch that waves skip when no lane needs it
-// (probability ~6e-7 per draw) instead of if-converting it into every step.
+// (probability ~6e-7 per draw) instead of if-converting it into every step. It is a real call,
+// not inlined: the f64 log/sqrt code and its constants then live in the callee (17 VGPRs, no
+// stack) instead of pinning registers for the whole step loop of the caller.
 // `scale` multiplies the result before it is rounded to f32.
-__device__ __forceinline__ float sde_inv_normal_tail(double q, double scale = 1.0) {
+__device__ __attribute__((noinline)) float sde_inv_normal_tail(double q, double scale = 1.0) {
   asm volatile("");
   const double t = sqrt(-2.0 * log(q));
   return (float)(scale * (t - ((0.010328 * t + 0.802853) * t + 2.515517) /
