@@ -3,8 +3,9 @@
 // Replaces sim::simulate's orchestration (src/sim/mod.rs:20-92): instead of a rayon loop over
 // scenarios that clones the universe, builds an RNG per scenario and steps it through boxed
 // trait objects, a plan generates the fused kernel for the universe (hipRTC, cached by source
-// text), uploads the time grid, initial row, Sobol tables and XOR masks once, and every run is
-// one kernel launch over all paths (plus a one-block reduction when terminal moments are asked).
+// text), uploads the time grid, initial row and Sobol tables (in XOR mode with the scramble
+// masks folded in) once, and every run is one kernel launch over all paths (plus a one-block
+// reduction when terminal moments are asked).
 #include <hip/hip_runtime_api.h>
 #include <hip/hiprtc.h>
 
