@@ -63,7 +63,9 @@ typedef struct sdesim_options {
   int32_t block_size;     /* 0 = default (256) */
   uint64_t seed;          /* path s uses seed + s (src/sim/mod.rs:56,65) */
   uint64_t scramble_seed; /* XOR masks = words of ChaCha8Rng::seed_from_u64(scramble_seed) */
-  int64_t sobol_first;    /* point index of path 0; < 0 = default (5 for shift, 0 for XOR) */
+  int64_t sobol_first;    /* point index of path 0; < 0 = default (5 for shift, 0 for XOR). In XOR
+                             mode a multiple of 64 lets each wave read its shared table words with
+                             scalar loads (the fast form); any other value is also exact */
 } sdesim_options;
 
 /* Fill `o` with the defaults: Euler, pseudo, f64, reference compat, filtration output. */
