@@ -222,6 +222,32 @@ def test_codegen_reference_rk_carries_probe_coefficients():
     assert src_tb[src_tb.index("void step"):].count("sde_sqrt(") == 6
 
 
+def test_codegen_fp32_reassociation_is_confined_to_the_step():
+    # FP32 steps may be reassociated by the compiler, but the state update goes through a strict
+    # addition (so results do not depend on how far the step loop is unrolled); FP64 steps keep
+    # the reference's evaluation order.
+    for scheme in (0, 1):
+        f32 = generate(HESTON, 33, scheme=scheme, precision=1, rng=1, scramble=1)
+        step = f32[f32.index("void step"):]
+        assert "#pragma clang fp reassociate(on)" in step
+        assert step.count("sde_add_strict(X[") == 2
+        f64 = generate(HESTON, 33, scheme=scheme, precision=0, rng=1, scramble=1)
+        assert "reassociate" not in f64 and "sde_add_strict" not in f64
+
+
+def test_codegen_wave_aligned_table_form():
+    # XOR mode with a first point index that is a multiple of 64 uses the wave-aligned table form
+    # (last template argument); other first indices and the shift mode use per-lane gathers.
+    def aligned(**kw):
+        src = generate(HESTON, 33, scheme=1, precision=1, rng=1, **kw)
+        call = re.search(r"sde_path_kernel<[^>]*>", src).group(0)
+        return call.endswith("true>")
+    assert aligned(scramble=1)
+    assert aligned(scramble=1, sobol_first=128)
+    assert not aligned(scramble=1, sobol_first=37)
+    assert not aligned(scramble=0, sobol_first=0)
+
+
 def test_codegen_payoff_and_names():
     src = generate(HESTON, 33, payoff="max(S - 100, 0)", scheme=1, precision=1, rng=1, scramble=1, outputs=2)
     assert "sdesim_rk_f32_sobolxor_term" in src and "X[0]" in src[src.index("payoff("):]
