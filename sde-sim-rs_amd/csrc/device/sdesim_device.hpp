@@ -189,15 +189,19 @@ __device__ __forceinline__ float sde_inv_normal_f32_x_scaled(sde_u32 yc, const f
   constexpr float A0 = (float)(2.515517 / c), A1 = 0.802853f, A2 = (float)(0.010328 * c);
   constexpr float B1 = (float)(1.432788 * c), B2 = (float)(0.189269 * c * c), B3 = (float)(0.001308 * c * c * c);
   const sde_u32 qi = yc ^ (sde_u32)((int)yc >> 31);
-  if (qi < 1233u) {
-    asm volatile("");
-    return tail[qi];
-  }
+  // The rational is evaluated for every lane (q >= 2^-33 keeps it finite) and the tail lanes then
+  // replace it: the fast path stays out of the branch, so the draws of a step share one basic
+  // block and can be interleaved.
   const float q = __builtin_fmaf((float)qi, 0x1.0p-32f, 0x1.0p-33f);
   const float t = __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(q));
   const float num = (A2 * t + A1) * t + A0;
   const float den = ((B3 * t + B2) * t + B1) * t + 1.0f;
-  return t - num * __builtin_amdgcn_rcpf(den);
+  float x = t - num * __builtin_amdgcn_rcpf(den);
+  if (qi < 1233u) {
+    asm volatile("");
+    x = tail[qi];
+  }
+  return x;
 }
 
 template <typename R> __device__ __forceinline__ R sde_psi(double p);

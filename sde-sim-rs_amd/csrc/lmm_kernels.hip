@@ -37,9 +37,11 @@ struct LmmStep {
   R eps[F];
 };
 
+// exp of an exponent given in the parameter rows' units: base 2 for f32 (the rows carry the
+// log2(e) factor, so this is one v_exp_f32), natural for f64.
 template <typename R>
-__device__ __forceinline__ R lmm_exp(R x) {
-  if constexpr (sizeof(R) == 4) return __expf(x);
+__device__ __forceinline__ R lmm_exp_row(R x) {
+  if constexpr (sizeof(R) == 4) return __builtin_amdgcn_exp2f(x);
   else return exp(x);
 }
 
@@ -62,21 +64,31 @@ __device__ __forceinline__ void lmm_noise(const LmmArgs& a, __amdgpu_buffer_rsrc
   }
 }
 
-// Model parameters live in a device buffer ([40] volatilities then [40][3] loadings, in the
-// kernel's precision) read through the constant address space. Each step reads them at index
-// i + off, where off = step * a.zero is 0 at run time but not known to the compiler, so the reads
-// are uniform scalar loads repeated per step instead of 160 loop-invariant values hoisted into
-// registers for the whole kernel.
+// Model parameters: the workgroup stages them once from the device buffer ([40] volatilities
+// then [40][3] loadings) into LDS, one row per forward with the step's constant products
+// precomputed, so a forward's parameters are two broadcast LDS reads. Each step reads them at
+// index i + off, where off = step * a.zero is 0 at run time but not known to the compiler: the
+// reads repeat per step (issued ahead of use within the step) instead of 320 loop-invariant
+// values being hoisted into registers for the whole kernel.
+template <typename R>
+struct LmmParRow {
+  R sig, ts;      // sigma_i, tau sigma_i
+  R b0, b1, b2;   // loadings B_i0..B_i2
+  R sd, ss, h;    // exponent terms sigma dt, sigma sqrt(dt), -sigma^2 dt / 2 (f32: times log2 e)
+};
+
 template <typename R>
 struct LmmPar {
-  const LmmArgs& a;
+  const LmmParRow<R>* rows;  // LDS
   int off;
-  __device__ __forceinline__ const __attribute__((address_space(4))) R* par() const {
-    return (const __attribute__((address_space(4))) R*)(sizeof(R) == 4 ? (const void*)a.parf : (const void*)a.pard);
-  }
-  __device__ __forceinline__ R vol(int i) const { return par()[i + off]; }
-  __device__ __forceinline__ R load(int i, int f) const { return par()[NF + (i + off) * F + f]; }
+  __device__ __forceinline__ LmmParRow<R> row(int i) const { return rows[i + off]; }
 };
+
+template <typename R>
+__device__ __forceinline__ R lmm_recip(R x) {
+  if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rcpf(x);  // v_rcp_f32, 1 ulp
+  else return R(1) / x;
+}
 
 // One log-Euler step of all alive forwards (forwards below `eta` are dead and unchanged).
 template <typename R>
@@ -86,23 +98,18 @@ __device__ __forceinline__ void lmm_step(const LmmPar<R>& tab, R (&L)[NF], const
 #pragma unroll
   for (int i = 0; i < NF; ++i) {
     if (i >= eta) {
-      const R sig = tab.vol(i);
-      const R b0 = tab.load(i, 0), b1 = tab.load(i, 1), b2 = tab.load(i, 2);
-      const R ai = tau * sig * L[i] / (R(1) + tau * L[i]);
+      const LmmParRow<R> pr = tab.row(i);
+      const R b0 = pr.b0, b1 = pr.b1, b2 = pr.b2;
+      const R ai = pr.ts * L[i] * lmm_recip<R>(R(1) + tau * L[i]);
       C[0] = C[0] + ai * b0;
       C[1] = C[1] + ai * b1;
       C[2] = C[2] + ai * b2;
-      const R drift = sig * (b0 * C[0] + b1 * C[1] + b2 * C[2]) - R(0.5) * sig * sig;
+      // (sigma (B.C) - sigma^2 / 2) dt + sigma sqrt(dt) (B.eps)
+      const R bc = b0 * C[0] + b1 * C[1] + b2 * C[2];
       const R w = b0 * eps[0] + b1 * eps[1] + b2 * eps[2];
-      L[i] = L[i] * lmm_exp<R>(drift * dt + sig * sdt * w);
+      L[i] = L[i] * lmm_exp_row<R>(pr.sd * bc + pr.ss * w + pr.h);
     }
   }
-}
-
-template <typename R>
-__device__ __forceinline__ R lmm_recip(R x) {
-  if constexpr (sizeof(R) == 4) return __builtin_amdgcn_rcpf(x);  // v_rcp_f32, 1 ulp
-  else return R(1) / x;
 }
 
 // Adjoint of lmm_step: L is the state before the step and Lp the state after it; on entry Lb is
@@ -114,15 +121,15 @@ __device__ __forceinline__ void lmm_step_adjoint(const LmmPar<R>& tab, const R (
 #pragma unroll
   for (int i = NF - 1; i >= 0; --i) {
     if (i >= eta) {
-      const R sig = tab.vol(i);
-      const R b0 = tab.load(i, 0), b1 = tab.load(i, 1), b2 = tab.load(i, 2);
+      const LmmParRow<R> pr = tab.row(i);
+      const R sig = pr.sig, b0 = pr.b0, b1 = pr.b1, b2 = pr.b2;
       const R db = Lb[i] * Lp[i] * dt;
       G[0] = G[0] + db * sig * b0;
       G[1] = G[1] + db * sig * b1;
       G[2] = G[2] + db * sig * b2;
       const R ab = b0 * G[0] + b1 * G[1] + b2 * G[2];
       const R rop = lmm_recip<R>(R(1) + tau * L[i]);
-      Lb[i] = Lb[i] * (Lp[i] * lmm_recip<R>(L[i])) + ab * tau * sig * (rop * rop);
+      Lb[i] = Lb[i] * (Lp[i] * lmm_recip<R>(L[i])) + ab * pr.ts * (rop * rop);
     }
   }
 }
@@ -137,6 +144,17 @@ __global__ void __launch_bounds__(256) sdesim_lmm(const LmmArgs a) {
   const long long sid = live ? s : 0;
   const R dt = (R)a.dt, sdt = (R)a.sdt, tau = (R)a.tau, K = (R)a.strike;
   const int e = a.expiry_index;
+
+  __shared__ LmmParRow<R> par_rows[NF];
+  if (threadIdx.x < NF) {
+    const R* par = sizeof(R) == 4 ? (const R*)(const void*)a.parf : (const R*)(const void*)a.pard;
+    const int i = (int)threadIdx.x;
+    const R sig = par[i];
+    const R un = sizeof(R) == 4 ? R(1.4426950408889634) : R(1);  // exponent unit: log2 e for f32
+    par_rows[i] = LmmParRow<R>{sig, tau * sig, par[NF + i * F], par[NF + i * F + 1], par[NF + i * F + 2],
+                               sig * dt * un, sig * sdt * un, R(-0.5) * sig * sig * dt * un};
+  }
+  __syncthreads();
 
   sde_u32 soff[NSL];
   {
@@ -160,7 +178,7 @@ __global__ void __launch_bounds__(256) sdesim_lmm(const LmmArgs a) {
     }
     R eps[F];
     lmm_noise<R>(a, rsrc, soff, n, eps);
-    lmm_step<R>(LmmPar<R>{a, n * a.zero}, L, eps, n / a.steps_per_period + 1, dt, sdt, tau);
+    lmm_step<R>(LmmPar<R>{par_rows, n * a.zero}, L, eps, n / a.steps_per_period + 1, dt, sdt, tau);
   }
 
   // Payoff and its derivative with respect to the terminal forwards.
@@ -223,14 +241,16 @@ __global__ void __launch_bounds__(256) sdesim_lmm(const LmmArgs a) {
         }
         R eps[F];
         lmm_noise<R>(a, rsrc, soff, n0 + k, eps);
-        lmm_step<R>(LmmPar<R>{a, (n0 + k) * a.zero}, Xc, eps, (n0 + k) / a.steps_per_period + 1, dt, sdt, tau);
+        lmm_step<R>(LmmPar<R>{par_rows, (n0 + k) * a.zero}, Xc, eps, (n0 + k) / a.steps_per_period + 1, dt, sdt,
+                    tau);
       }
       for (int k = len - 1; k >= 0; --k) {
         if (k + 1 < len) {
 #pragma unroll
           for (int i = 0; i < NF; ++i) Xc[i] = tape[((long long)k * NF + i) * S + sid];
         }
-        lmm_step_adjoint<R>(LmmPar<R>{a, (n0 + k) * a.zero}, Xc, Lp, Lb, (n0 + k) / a.steps_per_period + 1, dt, tau);
+        lmm_step_adjoint<R>(LmmPar<R>{par_rows, (n0 + k) * a.zero}, Xc, Lp, Lb, (n0 + k) / a.steps_per_period + 1,
+                            dt, tau);
 #pragma unroll
         for (int i = 0; i < NF; ++i) Lp[i] = Xc[i];
       }
