@@ -134,8 +134,11 @@ __device__ __forceinline__ void lmm_step_adjoint(const LmmPar<R>& tab, const R (
   }
 }
 
+// The f32 adjoint is held to 168 VGPRs (3 waves per SIMD instead of 2 at its natural 172, for a
+// few spilled registers): the extra wave hides the tape and LDS latency of the reverse sweep.
 template <typename R, bool ADJ>
-__global__ void __launch_bounds__(256) sdesim_lmm(const LmmArgs a) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADJ && sizeof(R) == 4 ? 3 : 1)))
+sdesim_lmm(const LmmArgs a) {
   constexpr int NQ = 2 + NF;
   const long long S = a.paths;
   const int N = a.steps;
