@@ -72,3 +72,36 @@ class ForwardRateModel:
         if h is not None and h.value:
             nat.lib.sdesim_lmm_free(h)
             self._h = None
+
+
+def _autograd_function():
+    import torch
+
+    class ForwardRatePrice(torch.autograd.Function):
+        """E[swaption payoff](L0) with the adjoint kernel's 40 curve sensitivities as the gradient.
+        The volatilities and loadings are inputs without gradients."""
+
+        @staticmethod
+        def forward(ctx, L0, vol, loadings, model):
+            r = model.evaluate(L0.detach().cpu().numpy(), vol.detach().cpu().numpy(),
+                               loadings.detach().cpu().numpy(), adjoint=True)
+            ctx.save_for_backward(torch.as_tensor(r["dL0"], dtype=L0.dtype, device=L0.device))
+            return torch.as_tensor(r["price"], dtype=L0.dtype, device=L0.device)
+
+        @staticmethod
+        def backward(ctx, g):
+            (dL0,) = ctx.saved_tensors
+            return g * dL0, None, None, None
+
+    return ForwardRatePrice
+
+
+_FORWARD_RATE_PRICE = None
+
+
+def forward_rate_price(L0, vol, loadings, model):
+    """Differentiable swaption price (torch tensors in, scalar tensor out; gradient for L0)."""
+    global _FORWARD_RATE_PRICE
+    if _FORWARD_RATE_PRICE is None:
+        _FORWARD_RATE_PRICE = _autograd_function()
+    return _FORWARD_RATE_PRICE.apply(L0, vol, loadings, model)

@@ -74,3 +74,16 @@ def test_lmm_f32_sensitivities_against_central_differences():
         worst = max(worst, err)
         assert err <= 1e-4, (i, adj["dL0"][i], fd)
     print("max relative error of 40 curve sensitivities vs FD: %.3g" % worst)
+
+
+def test_forward_rate_torch_autograd_function():
+    from sde_sim_rs.lmm import ForwardRateModel, forward_rate_config, forward_rate_price
+    c = forward_rate_config(seed=5)
+    print("case: lmm autograd function S=4096 N=16", flush=True)
+    m = ForwardRateModel(steps=16, paths=4096, precision="f64")
+    r = m.evaluate(c["L0"], c["vol"], c["loadings"], adjoint=True)
+    L0 = torch.tensor(c["L0"], requires_grad=True)
+    p = forward_rate_price(L0, torch.tensor(c["vol"]), torch.tensor(c["loadings"]), m)
+    assert abs(float(p.detach()) - r["price"]) <= 1e-15 * abs(r["price"])
+    (2.0 * p).backward()
+    np.testing.assert_allclose(L0.grad.numpy(), 2.0 * r["dL0"], rtol=1e-12)
