@@ -122,6 +122,19 @@ def test_xor_increment_counts(oracle, m, scheme):
     _close(f.transpose(2, 0, 1), ref)
 
 
+def test_xor_four_table_slices(oracle):
+    """Beyond 2^22 paths the Gray codes need 23 bits: four table slices, three of them read with
+    scalar loads in the wave-aligned form. 2^22 + 100 paths also leave a partial last wave."""
+    eqs = ["dX = (0.05 * X) * dt + (0.2 * X) * dW1"]
+    times = np.linspace(0.0, 1.0, 3)
+    S = (1 << 22) + 100
+    print("case: xor four slices S=%d" % S, flush=True)
+    _, f = _filtration(eqs, times, S, {"X": 100.0}, "sobol", "euler", seed=1, scramble="xor", scramble_seed=3)
+    ref = oracle.simulate(eqs, times, S, {"X": 100.0}, scheme="euler", rng="sobol", seed=1, scramble="xor",
+                          scramble_seed=3)
+    _close(f.transpose(2, 0, 1), ref)
+
+
 def test_fp32_xor_tail_increments(oracle):
     """FP32 XOR increments with qi = min(y, ~y) < 1233 (|z| > 5) read the host-tabulated tail. Over
     2^21 points of dimension 0 (van der Corput) some scrambled points land there; after one GBM
