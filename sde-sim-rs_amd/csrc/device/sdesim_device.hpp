@@ -167,13 +167,14 @@ __device__ __forceinline__ float sde_inv_normal_f32_x(sde_u32 yc) {
 // f32 with XOR uniforms: with c = sqrt(2 ln 2) and t = c t', t' = sqrt(-log2 q) (the negation
 // rides on the square root's source modifier), the A&S value is z = c x' with
 //   x' = t' - num'(t') / den'(t'),  num'(t') = num(c t') / c,  den'(t') = den(c t'),
-// and c and the sign of z go into the multiplier: one bit-select on the sign mask that also
-// forms qi (sign of u - 1/2) picks +-c sqrt(dt).
+// and c and the sign of z go into the multiplier: one bit-select on bit 31 of the uniform's word
+// (set when u < 1/2) picks +-c sqrt(dt).
 #define SDE_PSI_C 1.1774100225154747  // sqrt(2 ln 2)
 
-// |v| with the sign of u - 1/2 (bit 31 of yc): one v_bfi_b32 on the uniform's word. The sign
-// mask is materialised in a scalar register the compiler cannot see through (VOP3 takes no
-// literal on gfx9), so the select stays one instruction instead of an AND and an OR.
+// v (>= 0) with the sign of u - 1/2 (bit 31 of yc): one 3-input bit select on the uniform's word
+// (v_bitop3_b32 on gfx950). The sign mask is materialised in a scalar register the compiler
+// cannot see through (VOP3 takes no literal on gfx9), so the select stays one instruction
+// instead of an AND and an OR.
 __device__ __forceinline__ float sde_sign_select(sde_u32 yc, float v) {
   sde_u32 m;
   asm("s_mov_b32 %0, 0x80000000" : "=s"(m));
