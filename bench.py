@@ -74,6 +74,17 @@ def _time_fill(buf, steps, warmup):
     return a.elapsed_time(b) / steps
 
 
+def _profile_value(name, key):
+    """A `key: value` line of a committed rocprofv3 summary under profiles/, or None."""
+    try:
+        for ln in open(os.path.join(ROOT, "profiles", name)):
+            if ln.startswith(key + ":"):
+                return float(ln.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def _profiled_traffic(paths, nsteps):
     """HBM bytes per launch of the headline kernel from the committed rocprofv3 counter summary
     (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy); None for other sizes."""
@@ -261,6 +272,21 @@ def main():
         e = abs(term["f32"]["payoff_mean"] - term["f64"]["payoff_mean"]) / abs(term["f64"]["payoff_mean"])
         term["fp32_payoff_rel_err_vs_fp64"] = e
         term["fp32_over_fp64"] = term["f64"]["ms"] / term["f32"]["ms"]
+        # VALU roofline of the FP32 terminal-only kernel: wave64 VALU instructions per path-step
+        # from the committed counter summary, over the kernel's time here, against the MI355X peak
+        # of one wave64 FP32 VALU instruction per 2 clocks per SIMD (1024 SIMDs at 2.4 GHz);
+        # transcendentals run at a quarter of that rate and count 4 slots in the weighted form.
+        prof = "sdesim_rk_f32_sobolxor_term_heston_2p20x1024.md"
+        vi = _profile_value(prof, "valu_instructions_per_path_step")
+        tr = _profile_value(prof, "trans_instructions_per_path_step")
+        if vi is not None and tr is not None and (S, N) == (1 << 20, 1024):
+            peak = 1024 * 2.4e9 / 2
+            rate = vi * S * N / 64 / (term["f32"]["ms"] * 1e-3)
+            term["valu_roofline"] = {"bound": "valu", "achieved": rate, "peak": peak,
+                                     "unit": "wave64 VALU instructions/s", "frac": rate / peak,
+                                     "frac_trans_weighted": rate * (vi + 3 * tr) / vi / peak,
+                                     "valu_per_path_step": vi, "trans_per_path_step": tr,
+                                     "kernel": "sdesim_rk_f32_sobolxor_term"}
         line["terminal_only"] = term
     if batch:
         line["batched"] = batch
