@@ -94,8 +94,8 @@ struct LmmArgs {
   const float* parf;                  // the same in f32
   double dt, sdt, tau, strike;
   long long paths;
-  const sde_u32* sobol_tab;           // sliced Sobol tables (3 slices)
-  const sde_u32* xor_mask;            // [steps * 3]
+  const sde_u32* sobol_tab;           // slice 0 as [steps][64][3], the mask complements folded in
+  const sde_u32* sobol_up;            // slices 1-2 as [2][256][steps * 3]
   void* checkpoints;                  // [slots][40][paths] real, one slot per 16 steps
   void* tape;                         // [16][40][paths] real, the states of one segment
   double* partials;                   // [gridDim.x][42]
@@ -103,6 +103,6 @@ struct LmmArgs {
   int steps_per_period;
   int expiry_index;
   int sobol_tab_words;
+  int sobol_dims;                     // steps * 3, the row length of sobol_up
   int zero;                           // always 0: makes per-step parameter reads step-dependent
-  int pad_;
 };

@@ -45,20 +45,24 @@ __device__ __forceinline__ R lmm_exp_row(R x) {
   else return exp(x);
 }
 
-// The F uniforms of step n from the Sobol tables (digital XOR shift), as standard normals.
+// Where a lane's Sobol words live: its slice-0 entry (byte offset within a step's [64][F] block)
+// and the wave's rows of slices 1-2 (the wave's 64 point indices share those Gray-code bits).
+struct LmmSobol {
+  sde_u32 voff0;
+  sde_const_u32* urow[NSL - 1];
+};
+
+// The F uniforms of step n (digital XOR shift folded into the tables: the XOR of the slices is the
+// complement of the scrambled point), as standard normals: one vector load of the lane's F
+// slice-0 words and scalar loads of the wave's words of slices 1-2.
 template <typename R>
-__device__ __forceinline__ void lmm_noise(const LmmArgs& a, __amdgpu_buffer_rsrc_t rsrc, const sde_u32 (&soff)[NSL],
-                                          int n, R (&eps)[F]) {
-  const sde_u32 stride = 64u + 256u * (NSL - 1);
+__device__ __forceinline__ void lmm_noise(__amdgpu_buffer_rsrc_t rsrc, const LmmSobol& sb, int n, R (&eps)[F]) {
+  sde_u32 w[F];
+  sde_load_words<F>(rsrc, (int)sb.voff0, n * (64 * F * 4), w);
 #pragma unroll
   for (int f = 0; f < F; ++f) {
-    const int d = n * F + f;
-    const int base = (int)((sde_u32)d * stride * 4u);
-    sde_u32 x = a.xor_mask[d];
-#pragma unroll
-    for (int sl = 0; sl < NSL; ++sl) x ^= (sde_u32)__builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)soff[sl], base, 0);
     SdeUnifX u;
-    u.yc = ~x;
+    u.yc = w[f] ^ sb.urow[0][n * F + f] ^ sb.urow[1][n * F + f];
     u.tail = nullptr;
     eps[f] = sde_psi<R>(u);
   }
@@ -159,12 +163,18 @@ sdesim_lmm(const LmmArgs a) {
   }
   __syncthreads();
 
-  sde_u32 soff[NSL];
+  LmmSobol sb;
   {
+    // Point index = path index; blocks of 256 make every wave one aligned run of 64 indices.
+    // Lane 0 has the wave's smallest index: if it is past the last path so is every lane of the
+    // wave, whose values are then never used.
     const sde_u64 g = (sde_u64)sid ^ ((sde_u64)sid >> 1);
-    sde_sobol_offsets<NSL>(g, soff);
+    sb.voff0 = (sde_u32)(g & 63u) * (sde_u32)(F * 4);
 #pragma unroll
-    for (int sl = 0; sl < NSL; ++sl) soff[sl] *= 4u;
+    for (int sl = 1; sl < NSL; ++sl) {
+      const sde_u32 e = __builtin_amdgcn_readfirstlane((sde_u32)(g >> (6 + 8 * (sl - 1))) & 255u);
+      sb.urow[sl - 1] = (sde_const_u32*)a.sobol_up + ((size_t)(sl - 1) * 256u + e) * (size_t)a.sobol_dims;
+    }
   }
   __amdgpu_buffer_rsrc_t rsrc =
       __builtin_amdgcn_make_buffer_rsrc((void*)a.sobol_tab, (short)0, a.sobol_tab_words * 4, 0x00020000);
@@ -180,7 +190,7 @@ sdesim_lmm(const LmmArgs a) {
       for (int i = 0; i < NF; ++i) ck[(slot * NF + i) * S + sid] = L[i];
     }
     R eps[F];
-    lmm_noise<R>(a, rsrc, soff, n, eps);
+    lmm_noise<R>(rsrc, sb, n, eps);
     lmm_step<R>(LmmPar<R>{par_rows, n * a.zero}, L, eps, n / a.steps_per_period + 1, dt, sdt, tau);
   }
 
@@ -243,7 +253,7 @@ sdesim_lmm(const LmmArgs a) {
           for (int i = 0; i < NF; ++i) tape[((long long)k * NF + i) * S + sid] = Xc[i];
         }
         R eps[F];
-        lmm_noise<R>(a, rsrc, soff, n0 + k, eps);
+        lmm_noise<R>(rsrc, sb, n0 + k, eps);
         lmm_step<R>(LmmPar<R>{par_rows, (n0 + k) * a.zero}, Xc, eps, (n0 + k) / a.steps_per_period + 1, dt, sdt,
                     tau);
       }

@@ -300,15 +300,15 @@ const float* tail_table() {
   return d;
 }
 
-// Shared with the other runtime files: device Sobol tables and ChaCha8 words.
-const unsigned int* sdesim_device_sobol_tables(int dims, int nslices, int* words) {
-  SobolTables t = sobol_tables(dims, nslices);
-  if (words) *words = t.dims * (64 + 256 * (nslices - 1));
-  return t.dev;
-}
-
+// Shared with the other runtime files: ChaCha8 words and host Sobol tables.
 void sdesim_chacha8_words(unsigned long long seed, size_t n, std::vector<unsigned int>& out) {
   chacha8_words(seed, n, out);
+}
+
+// Host copy of the sliced tables [dims][64 + 256 (nslices - 1)], for runtimes that lay them out
+// for their own access pattern.
+void sdesim_host_sobol_tables(int dims, int nslices, std::vector<unsigned int>& out) {
+  build_sobol_tables(dims, nslices, out);
 }
 
 struct sdesim_plan {

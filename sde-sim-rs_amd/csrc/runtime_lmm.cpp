@@ -14,7 +14,7 @@
 hipError_t sdesim_launch_lmm(int f32, int adjoint, int grid, const LmmArgs& a, hipStream_t stream);
 hipError_t sdesim_launch_sum_partials(const double* partials, int nblocks, int nq, double* out, hipStream_t stream);
 int sdesim_set_error(int code, const std::string& msg);
-const unsigned int* sdesim_device_sobol_tables(int dims, int nslices, int* words);
+void sdesim_host_sobol_tables(int dims, int nslices, std::vector<unsigned int>& out);
 void sdesim_chacha8_words(unsigned long long seed, size_t n, std::vector<unsigned int>& out);
 
 namespace {
@@ -43,9 +43,9 @@ int guard(F&& f) {
 struct sdesim_lmm {
   sdesim_lmm_spec spec{};
   long long grid = 0;
-  const unsigned int* tab = nullptr;
+  unsigned int* tab = nullptr;    // slice 0, [steps][64][3], mask complements folded in
+  unsigned int* tabup = nullptr;  // slices 1-2, [2][256][steps * 3]
   int tab_words = 0;
-  unsigned int* masks = nullptr;
   void* ckpt = nullptr;
   void* tape = nullptr;
   double* pard = nullptr;
@@ -53,7 +53,7 @@ struct sdesim_lmm {
   double* partials = nullptr;
   double* sums = nullptr;
   ~sdesim_lmm() {
-    for (void* p : {(void*)masks, ckpt, tape, (void*)pard, (void*)parf, (void*)partials, (void*)sums})
+    for (void* p : {(void*)tab, (void*)tabup, ckpt, tape, (void*)pard, (void*)parf, (void*)partials, (void*)sums})
       if (p) (void)hipFree(p);
   }
 };
@@ -74,12 +74,32 @@ int sdesim_lmm_create(const sdesim_lmm_spec* spec, sdesim_lmm** out) {
     std::unique_ptr<sdesim_lmm> m(new sdesim_lmm());
     m->spec = s;
     m->grid = (s.paths + 255) / 256;
+    // Sobol tables laid out for the wave-aligned read (paths 0 .. S-1: each wave's 64 point
+    // indices share their Gray-code bits above bit 5): slice 0 as [steps][64][3] with the
+    // complement of each dimension's XOR mask folded in (one vector load per lane and step), and
+    // slices 1-2 as [2][256][dims] (consecutive words of a step for the wave's entry, read with
+    // scalar loads). Paths stay below 2^22, so three slices cover the Gray codes.
     const int dims = s.steps * kF;
-    m->tab = sdesim_device_sobol_tables(dims, 3, &m->tab_words);
-    std::vector<unsigned int> masks;
+    const size_t stride = 64 + 256 * 2;
+    std::vector<unsigned int> tab, masks;
+    sdesim_host_sobol_tables(dims, 3, tab);
     sdesim_chacha8_words(s.scramble_seed, (size_t)dims, masks);
-    lk(hipMalloc(&m->masks, sizeof(unsigned int) * dims), "hipMalloc(masks)");
-    lk(hipMemcpy(m->masks, masks.data(), sizeof(unsigned int) * dims, hipMemcpyHostToDevice), "hipMemcpy(masks)");
+    std::vector<unsigned int> t0((size_t)s.steps * 64 * kF), up((size_t)2 * 256 * dims);
+    for (int n = 0; n < s.steps; ++n)
+      for (int e = 0; e < 64; ++e)
+        for (int f = 0; f < kF; ++f) {
+          const int d = n * kF + f;
+          t0[((size_t)n * 64 + e) * kF + f] = tab[(size_t)d * stride + e] ^ ~masks[d];
+        }
+    for (int sl = 1; sl < 3; ++sl)
+      for (int e = 0; e < 256; ++e)
+        for (int d = 0; d < dims; ++d)
+          up[((size_t)(sl - 1) * 256 + e) * dims + d] = tab[(size_t)d * stride + 64 + 256 * (sl - 1) + e];
+    lk(hipMalloc(&m->tab, t0.size() * sizeof(unsigned int)), "hipMalloc(sobol tables)");
+    lk(hipMemcpy(m->tab, t0.data(), t0.size() * sizeof(unsigned int), hipMemcpyHostToDevice), "hipMemcpy");
+    lk(hipMalloc(&m->tabup, up.size() * sizeof(unsigned int)), "hipMalloc(sobol tables)");
+    lk(hipMemcpy(m->tabup, up.data(), up.size() * sizeof(unsigned int), hipMemcpyHostToDevice), "hipMemcpy");
+    m->tab_words = (int)t0.size();
     const size_t rb = s.precision == SDESIM_F32 ? 4 : 8;
     const size_t slots = (size_t)(s.steps + kSeg - 1) / kSeg;
     lk(hipMalloc(&m->ckpt, slots * kNF * (size_t)s.paths * rb), "hipMalloc(checkpoints)");
@@ -124,7 +144,8 @@ int sdesim_lmm_eval(sdesim_lmm* m, const double* L0, const double* vol, const do
     a.paths = s.paths;
     a.sobol_tab = m->tab;
     a.sobol_tab_words = m->tab_words;
-    a.xor_mask = m->masks;
+    a.sobol_up = m->tabup;
+    a.sobol_dims = s.steps * kF;
     a.checkpoints = m->ckpt;
     a.tape = m->tape;
     a.partials = m->partials;
