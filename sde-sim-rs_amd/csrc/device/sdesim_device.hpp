@@ -135,6 +135,7 @@ __device__ __forceinline__ float sde_inv_normal_f32(double p) {
 struct SdeUnifX {
   sde_u32 yc;
   const float* tail;  // the f32 increment's table of the 1233 tail values (|z| > 5), or null
+  sde_u32 sm, sm2;    // 0x80000000, the sign-bit mask of the f32 selects, in two vector registers
 };
 
 __device__ __forceinline__ double sde_uval(double u) { return u; }
@@ -142,17 +143,27 @@ __device__ __forceinline__ double sde_uval(SdeUnifX u) { return ((double)(~u.yc)
 __device__ __forceinline__ bool sde_gt_half(double u) { return u > 0.5; }
 __device__ __forceinline__ bool sde_gt_half(SdeUnifX u) { return (int)u.yc >= 0; }
 
-// f32 inverse normal of (y + 0.5) 2^-32 formed in the integer domain from yc = ~y: q = min(u,
-// 1 - u) is (qi + 0.5) 2^-32 with qi = min(y, ~y) = yc ^ (yc >> 31, arithmetic). qi < 1233
-// (|z| > 5) takes the f64 path.
+// q = min(u, 1 - u) of u = (y + 0.5) 2^-32 from yc = ~y, in f32. q is (qi + 0.5) 2^-32 with
+// qi = min(y, ~y) = yc ^ (yc >> 31, arithmetic). The word is not folded first: as a signed
+// integer, yc + 0.5 is qi + 0.5 when bit 31 is clear (qi = yc) and -(qi + 0.5) when it is set
+// (yc - 2^32 = -(~yc + 1)), so q = |(float)(int)yc 2^-32 + 2^-33|, with the absolute value on the
+// consumer's source modifier. That is exact for qi < 2^23 and otherwise within the rounding of
+// the 24-bit conversion. q < 1233 2^-32 holds exactly when qi < 1233 (|z| > 5).
+__device__ __forceinline__ float sde_q_f32(sde_u32 yc) {
+  return __builtin_fabsf(__builtin_fmaf((float)(int)yc, 0x1.0p-32f, 0x1.0p-33f));
+}
+__device__ __forceinline__ bool sde_q_in_tail(float q) { return q < 1233.0f * 0x1.0p-32f; }
+__device__ __forceinline__ sde_u32 sde_q_index(sde_u32 yc) { return yc ^ (sde_u32)((int)yc >> 31); }
+
+// f32 inverse normal of (y + 0.5) 2^-32 from yc = ~y; the tail qi < 1233 (|z| > 5) takes the
+// f64 path.
 __device__ __forceinline__ float sde_inv_normal_f32_x(sde_u32 yc) {
   const bool upper = (int)yc >= 0;
-  const sde_u32 qi = yc ^ (sde_u32)((int)yc >> 31);
+  const float q = sde_q_f32(yc);
   float x;
-  if (qi < 1233u) {
-    x = sde_inv_normal_tail(((double)qi + 0.5) * 0x1.0p-32);
+  if (sde_q_in_tail(q)) {
+    x = sde_inv_normal_tail(((double)sde_q_index(yc) + 0.5) * 0x1.0p-32);
   } else {
-    const float q = __builtin_fmaf((float)qi, 0x1.0p-32f, 0x1.0p-33f);
     const float t = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(q));
     const float num = (0.010328f * t + 0.802853f) * t + 2.515517f;
     const float den = ((0.001308f * t + 0.189269f) * t + 1.432788f) * t + 1.0f;
@@ -172,12 +183,17 @@ __device__ __forceinline__ float sde_inv_normal_f32_x(sde_u32 yc) {
 #define SDE_PSI_C 1.1774100225154747  // sqrt(2 ln 2)
 
 // v (>= 0) with the sign of u - 1/2 (bit 31 of yc): one 3-input bit select on the uniform's word
-// (v_bitop3_b32 on gfx950). The sign mask is materialised in a scalar register the compiler
-// cannot see through (VOP3 takes no literal on gfx9), so the select stays one instruction
-// instead of an AND and an OR.
-__device__ __forceinline__ float sde_sign_select(sde_u32 yc, float v) {
+// (v_and_or_b32 or v_bitop3_b32). The mask m = 0x80000000 comes from a vector register that the
+// compiler cannot see through (VOP3 takes no literal on gfx9), so a wave-uniform v (sqrt(dt) in
+// a scalar register) is the select's only scalar operand. The Wiener increment and the RK sign
+// of the same uniform use different mask registers (SdeUnifX::sm and sm2): with one, the shared
+// (yc & m) would be formed once and each select would become an OR plus that AND.
+__device__ __forceinline__ sde_u32 sde_sign_mask() {
   sde_u32 m;
-  asm("s_mov_b32 %0, 0x80000000" : "=s"(m));
+  asm volatile("v_mov_b32 %0, 0x80000000" : "=v"(m));
+  return m;
+}
+__device__ __forceinline__ float sde_sign_select(sde_u32 yc, float v, sde_u32 m) {
   return __builtin_bit_cast(float, (m & yc) | (~m & __builtin_bit_cast(sde_u32, v)));
 }
 
@@ -189,18 +205,17 @@ __device__ __forceinline__ float sde_inv_normal_f32_x_scaled(sde_u32 yc, const f
   constexpr double c = SDE_PSI_C;
   constexpr float A0 = (float)(2.515517 / c), A1 = 0.802853f, A2 = (float)(0.010328 * c);
   constexpr float B1 = (float)(1.432788 * c), B2 = (float)(0.189269 * c * c), B3 = (float)(0.001308 * c * c * c);
-  const sde_u32 qi = yc ^ (sde_u32)((int)yc >> 31);
   // The rational is evaluated for every lane (q >= 2^-33 keeps it finite) and the tail lanes then
   // replace it: the fast path stays out of the branch, so the draws of a step share one basic
   // block and can be interleaved.
-  const float q = __builtin_fmaf((float)qi, 0x1.0p-32f, 0x1.0p-33f);
+  const float q = sde_q_f32(yc);
   const float t = __builtin_amdgcn_sqrtf(-__builtin_amdgcn_logf(q));
   const float num = (A2 * t + A1) * t + A0;
   const float den = ((B3 * t + B2) * t + B1) * t + 1.0f;
   float x = t - num * __builtin_amdgcn_rcpf(den);
-  if (qi < 1233u) {
+  if (sde_q_in_tail(q)) {
     asm volatile("");
-    x = tail[qi];
+    x = tail[sde_q_index(yc)];
   }
   return x;
 }
@@ -216,12 +231,12 @@ __device__ __forceinline__ double sde_dw(double u, double sdt) { return sdt * sd
 __device__ __forceinline__ float sde_dw(double u, float sdt) { return sdt * sde_inv_normal_f32(u); }
 __device__ __forceinline__ double sde_dw(SdeUnifX u, double sdt) { return sdt * sde_inv_normal(sde_uval(u)); }
 __device__ __forceinline__ float sde_dw(SdeUnifX u, float sdt) {
-  return sde_sign_select(u.yc, sdt * (float)SDE_PSI_C) * sde_inv_normal_f32_x_scaled(u.yc, u.tail);
+  return sde_sign_select(u.yc, sdt * (float)SDE_PSI_C, u.sm) * sde_inv_normal_f32_x_scaled(u.yc, u.tail);
 }
 
 template <typename R> __device__ __forceinline__ R sde_signed(double u, R sdt) { return (u > 0.5 ? R(1) : R(-1)) * sdt; }
 __device__ __forceinline__ double sde_signed(SdeUnifX u, double sdt) { return (sde_gt_half(u) ? 1.0 : -1.0) * sdt; }
-__device__ __forceinline__ float sde_signed(SdeUnifX u, float sdt) { return sde_sign_select(u.yc, sdt); }
+__device__ __forceinline__ float sde_signed(SdeUnifX u, float sdt) { return sde_sign_select(u.yc, sdt, u.sm2); }
 
 // Sequential inverse Poisson CDF capped at k = 200 (f64, as the reference).
 __device__ __forceinline__ double sde_inv_poisson(double u, double lambda) {
@@ -563,6 +578,9 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
   }
   SdeGrid<real> g{};
   if (N > 0) g = sde_grid<real, UG>(a, 0);
+  const bool sel_masks = RNG == SDE_RNG_SOBOL_XOR && sizeof(real) == 4;
+  const sde_u32 smask = sel_masks ? sde_sign_mask() : 0x80000000u;
+  const sde_u32 smask2 = sel_masks ? sde_sign_mask() : 0x80000000u;
 
   for (int n0 = 0; n0 < N; n0 += U) {
     sde_u32 W[PPL][USE_CHACHA ? NB * 16 : 1];
@@ -602,6 +620,8 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
             if constexpr (RNG == SDE_RNG_SOBOL_XOR) {
               u[j][k].yc = nx[j][k];
               u[j][k].tail = a.tail_tab;
+              u[j][k].sm = smask;
+              u[j][k].sm2 = smask2;
             } else {
               u[j][k] = v;
             }

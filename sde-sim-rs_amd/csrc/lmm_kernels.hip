@@ -64,6 +64,7 @@ __device__ __forceinline__ void lmm_noise(__amdgpu_buffer_rsrc_t rsrc, const Lmm
     SdeUnifX u;
     u.yc = w[f] ^ sb.urow[0][n * F + f] ^ sb.urow[1][n * F + f];
     u.tail = nullptr;
+    u.sm = u.sm2 = 0x80000000u;
     eps[f] = sde_psi<R>(u);
   }
 }

@@ -294,3 +294,28 @@ def test_empty_universe_returns_empty_table():
     t = sde_sim_rs.simulate([], [0.0, 1.0], 3, {}, "pseudo", "euler")
     tab = t if hasattr(t, "column") else t.to_arrow()
     assert tab.num_rows == 0 and tab.column_names == ["scenario", "time", "process_name", "value"]
+
+
+def test_fp32_q_from_signed_word():
+    """The f32 device forms q = min(u, 1 - u) of u = (y + 0.5) 2^-32 from yc = ~y without folding
+    the word: q = |fma((float)(int)yc, 2^-32, 2^-33)|, and a lane is in the |z| > 5 tail when
+    q < 1233 2^-32. Checked here against the definition qi = min(y, ~y), q = (qi + 0.5) 2^-32, with
+    numpy's round-to-nearest f32 conversion and a single rounding of the exact f64 sum (the fma):
+    the tail test must agree with qi < 1233 exactly, q must be exact for qi < 2^23, and within one
+    f32 rounding of the conversion otherwise."""
+    rng = np.random.default_rng(5)
+    edge = np.arange(0, 4096, dtype=np.uint64)
+    yc = np.concatenate([edge, 0xFFFFFFFF - edge, 0x7FFFFFFF - edge, 0x80000000 + edge,
+                         (1 << 23) + np.arange(-64, 64, dtype=np.int64).astype(np.uint64),
+                         0xFFFFFFFF - (1 << 23) + np.arange(-64, 64, dtype=np.int64).astype(np.uint64),
+                         rng.integers(0, 1 << 32, 200000, dtype=np.uint64)]).astype(np.uint32)
+    f = yc.view(np.int32).astype(np.float32)
+    q = np.abs((f.astype(np.float64) * 2.0**-32 + 2.0**-33).astype(np.float32))
+    y = ~yc
+    qi = np.minimum(y, ~y).astype(np.float64)
+    exact = (qi + 0.5) * 2.0**-32
+    in_tail = q < np.float32(1233.0 * 2.0**-32)
+    np.testing.assert_array_equal(in_tail, qi < 1233)
+    small = qi < (1 << 23)
+    np.testing.assert_array_equal(q[small].astype(np.float64), exact[small])
+    assert np.all(np.abs(q.astype(np.float64) - exact) <= 2.0**-23 * exact)
