@@ -147,8 +147,11 @@ __device__ __forceinline__ bool sde_gt_half(SdeUnifX u) { return (int)u.yc >= 0;
 // qi = min(y, ~y) = yc ^ (yc >> 31, arithmetic). The word is not folded first: as a signed
 // integer, yc + 0.5 is qi + 0.5 when bit 31 is clear (qi = yc) and -(qi + 0.5) when it is set
 // (yc - 2^32 = -(~yc + 1)), so q = |(float)(int)yc 2^-32 + 2^-33|, with the absolute value on the
-// consumer's source modifier. That is exact for qi < 2^23 and otherwise within the rounding of
-// the 24-bit conversion. q < 1233 2^-32 holds exactly when qi < 1233 (|z| > 5).
+// consumer's source modifier. That is exact for qi < 2^23. Above, the 24-bit conversion rounds
+// qi on one half and qi + 1 on the other: q is within one f32 rounding of its value, with a
+// bias of half a unit of 2^-32 whose sign differs between the halves (the FP32 call payoff of
+// the 2^20-path Heston run moves by 3e-8 relative against FP64). q < 1233 2^-32 holds exactly
+// when qi < 1233 (|z| > 5).
 __device__ __forceinline__ float sde_q_f32(sde_u32 yc) {
   return __builtin_fabsf(__builtin_fmaf((float)(int)yc, 0x1.0p-32f, 0x1.0p-33f));
 }
