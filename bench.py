@@ -128,6 +128,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true", help="run a 1 GiB device copy first (HBM counter calibration)")
     ap.add_argument("--only", default="", help="comma list of legs to run: f32,f64,term,batch,adjoint,lmm")
+    ap.add_argument("--basket-storage", default="auto", choices=("auto", "tape", "checkpoints"),
+                    help="state storage of the basket adjoint")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: paths are independent and the multi-GPU form "
@@ -204,7 +206,8 @@ def main():
         from sde_sim_rs.adjoint import Basket, basket_config
         c = basket_config(seed=3)
         AS, AN = 1 << 22, 365
-        bk = Basket(8, AN, AS, horizon=1.0, strike=100.0, width=0.5, seed=3, precision="f32")
+        bk = Basket(8, AN, AS, horizon=1.0, strike=100.0, width=0.5, seed=3, precision="f32",
+                    storage=args.basket_storage)
         times = {}
         for mode in (False, True):
             for _ in range(max(1, args.warmup)):
@@ -218,7 +221,8 @@ def main():
                "greeks": 52, "adjoint_ms": times[True], "price_ms": times[False],
                "adjoint_over_price": times[True] / times[False],
                "path_steps_per_s_adjoint": AS * AN / (times[True] * 1e-3), "price": r["price"],
-               "checkpoint_every": 19, "checkpoint_bytes": ((AN + 18) // 19) * 8 * AS * 4}
+               "storage": bk.storage,
+               "state_bytes": (AN + 1) * 8 * AS * 4 if bk.storage == "tape" else ((AN + 18) // 19) * 8 * AS * 4}
         del bk
 
     lmm = None

@@ -202,11 +202,22 @@ typedef struct sdesim_basket_spec {
   double width;
   uint64_t seed;      /* path s uses seed + s */
   int32_t precision;  /* SDESIM_F32 | SDESIM_F64 (path arithmetic; path sums are f64) */
-  int32_t pad_;
+  int32_t storage;    /* adjoint state storage: SDESIM_STORAGE_AUTO, _TAPE or _CHECKPOINTS */
 } sdesim_basket_spec;
+
+/* Adjoint state storage. TAPE keeps every state in device memory (steps * assets * paths values
+ * in the path precision, one write and one read each; needs assets * paths * size < 2^31).
+ * CHECKPOINTS keeps every 19th state and recomputes the rest during the reverse sweep (one more
+ * forward pass of arithmetic, 19 times less memory). AUTO takes the tape when it fits in half of
+ * the free device memory at creation, otherwise checkpoints. */
+#define SDESIM_STORAGE_AUTO 0
+#define SDESIM_STORAGE_TAPE 1
+#define SDESIM_STORAGE_CHECKPOINTS 2
 
 int sdesim_basket_create(const sdesim_basket_spec* spec, sdesim_basket** out);
 void sdesim_basket_free(sdesim_basket* b);
+/* The storage the handle uses: SDESIM_STORAGE_TAPE or SDESIM_STORAGE_CHECKPOINTS. */
+int32_t sdesim_basket_storage(const sdesim_basket* b);
 /* chol: lower Cholesky factor of the correlation, row-major [assets][assets]. out[0] = mean
  * payoff, out[1] = its sample variance; with adjoint != 0 also the path-averaged derivatives
  * d/dx0[assets], d/dmu[assets], d/dsigma[assets], d/dL[assets (assets + 1) / 2] (row-major lower

@@ -9,10 +9,16 @@
 // plain call max(B - K, 0) when width == 0.
 //
 // Adjoint (no reference counterpart): pathwise derivatives of the payoff with respect to X0, mu,
-// sigma and L, per path, reduced over paths in f64. The forward pass stores the state every SEG
-// steps (checkpoints in HBM, [slot][asset][path]); the reverse sweep walks the segments
-// backwards, recomputes each segment's states from its checkpoint and then steps back through
-// it. A reverse step reads the step's noise off the two states instead of regenerating it:
+// sigma and L, per path, reduced over paths in f64. The reverse sweep needs every state X_n. Two
+// storage forms:
+//  * tape (MODE 2): the forward pass writes every state to HBM ([step][asset][path], 4 or 8 bytes
+//    each; 49 GB for 8 assets x 365 steps x 2^22 paths in f32) and the reverse sweep reads them
+//    back, one step ahead of its arithmetic;
+//  * checkpoints (MODE 1): the forward pass stores the state every SEG steps ([slot][asset][path]);
+//    the reverse sweep walks the segments backwards, recomputes each segment's states from its
+//    checkpoint into registers and then steps back through it (one more forward pass of
+//    arithmetic, for SEG times less memory).
+// A reverse step reads the step's noise off the two states instead of regenerating it:
 //   g_n = X_{n+1} / X_n = 1 + mu dt + sigma sqrt(dt) Z_n,  d_n = (X_{n+1} - X_n) - X_n mu dt
 // (X_{n+1} - X_n is exact, the two being within a factor 2; forming 1 + mu dt first would round
 // mu dt ~ 2e-4 to the spacing of 1 and bias every recovered Z_n)
@@ -84,33 +90,200 @@ __device__ __forceinline__ R basket_recip(R x) {
   else return R(1) / x;
 }
 
+// One reverse step: x = X_n, xp = X_{n+1} (replaced by x on return), with the adjoint state
+// Xb = Xbar_{n+1} (replaced by Xbar_n) and the gradient sums gm, gs (sigma's, times sigma), gL.
+template <typename R, int D>
+__device__ __forceinline__ void basket_reverse_step(const R (&x)[D], R (&xp)[D], R (&Xb)[D], R (&gm)[D], R (&gs)[D],
+                                                    R (&gL)[D * (D + 1) / 2], const R (&L)[D][D], const R (&rLd)[D],
+                                                    const R (&mdt)[D], const R (&rsd)[D], const R (&sig)[D], R sdt,
+                                                    R dt) {
+  R Zb[D], Z[D], eps[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    const R xb = Xb[i], xi = x[i], xpi = xp[i];
+    const R rx = basket_recip<R>(xi);
+    const R d = (xpi - xi) - xi * mdt[i];
+    gm[i] = gm[i] + xb * xi * dt;
+    gs[i] = gs[i] + xb * d;
+    Zb[i] = xb * xi * sig[i] * sdt;
+    Z[i] = d * rx * rsd[i];
+    Xb[i] = xb * (xpi * rx);
+    xp[i] = xi;
+  }
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    R r = Z[i];
+#pragma unroll
+    for (int kk = 0; kk < i; ++kk) r = r - L[i][kk] * eps[kk];
+    eps[i] = r * rLd[i];
+  }
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+#pragma unroll
+    for (int kk = 0; kk <= i; ++kk) {
+      const int q = i * (i + 1) / 2 + kk;
+      gL[q] = gL[q] + Zb[i] * eps[kk];
+    }
+  }
+}
+
+// The tape's states of step n: rows [n][asset][path] under one buffer resource (D * S * sizeof(R)
+// bytes, below 2^31, checked by the host); the path's byte offset is the vector offset, the
+// asset's row the scalar offset. Lanes past the last path use vector offset 2^31, beyond the
+// resource: their stores are dropped and their loads return 0 (values that are never reduced).
+template <typename R, int D>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t basket_tape_rsrc(void* tape, int n, long long S) {
+  // n is wave-uniform; readfirstlane keeps the resource in scalar registers even where the index
+  // was formed by a vector instruction (a clamped subtraction), which would otherwise make the
+  // compiler wrap every access in a loop over distinct resources.
+  n = __builtin_amdgcn_readfirstlane(n);
+  const long long step_bytes = (long long)D * S * (long long)sizeof(R);
+  return __builtin_amdgcn_make_buffer_rsrc((void*)((char*)tape + (long long)n * step_bytes), (short)0, (int)step_bytes,
+                                           0x00020000);
+}
+template <typename R, int D>
+__device__ __forceinline__ void basket_tape_store(void* tape, int n, long long S, sde_u32 voff, const R (&X)[D]) {
+  const __amdgpu_buffer_rsrc_t rs = basket_tape_rsrc<R, D>(tape, n, S);
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    const int soff = (int)((long long)i * S * (long long)sizeof(R));
+    if constexpr (sizeof(R) == 4)
+      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(sde_u32, (float)X[i]), rs, (int)voff, soff, 2);
+    else
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(SdeU32x2, (double)X[i]), rs, (int)voff, soff, 2);
+  }
+}
+template <typename R, int D>
+__device__ __forceinline__ void basket_tape_load(void* tape, int n, long long S, sde_u32 voff, R (&X)[D]) {
+  const __amdgpu_buffer_rsrc_t rs = basket_tape_rsrc<R, D>(tape, n, S);
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    const int soff = (int)((long long)i * S * (long long)sizeof(R));
+    if constexpr (sizeof(R) == 4)
+      X[i] = __builtin_bit_cast(float, (sde_u32)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, soff, 2));
+    else
+      X[i] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, soff, 2));
+  }
+}
+
 template <typename R>
 __device__ __forceinline__ R basket_softplus(R x) {
   // log(1 + e^x) without overflow: max(x, 0) + log1p(exp(-|x|))
   return (x > R(0) ? x : R(0)) + (R)log1p(exp(-(double)fabs((double)x)));
 }
 
-template <typename R, int D, int SEG, bool ADJ>
+// Payoff of the weighted basket at the final state and its derivative with respect to B.
+template <typename R, int D>
+__device__ __forceinline__ void basket_payoff(const R (&X)[D], const R (&w)[D], R K, R width, R& pay, R& dpay) {
+  R B = R(0);
+#pragma unroll
+  for (int i = 0; i < D; ++i) B = B + w[i] * X[i];
+  if (width > R(0)) {
+    const R x = (B - K) / width;
+    pay = width * basket_softplus<R>(x);
+    dpay = (R)(1.0 / (1.0 + exp(-(double)x)));
+  } else {
+    pay = B > K ? B - K : R(0);
+    dpay = B > K ? R(1) : R(0);
+  }
+}
+
+// The reverse sweep's parameters (wave-uniform) and per-path adjoint state.
+template <typename R, int D>
+struct BasketReverse {
+  static constexpr int NL = BasketConst<D>::NL;
+  R L[D][D], sig[D], mdt[D], rsd[D], rLd[D];  // L, sigma, mu dt, 1 / (sigma sqrt(dt)), 1 / L_ii
+  R dt, sdt;
+  R Xb[D], gm[D], gs[D], gL[NL];
+  R Xp[D];  // the state after the step being reversed: X_N first, then each state stepped back to
+
+  __device__ __forceinline__ BasketReverse(const BasketArgs& a, const R (&XN)[D], R dpay) {
+    dt = (R)a.dt;
+    sdt = (R)a.sdt;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) L[i][k] = (R)a.chol[i * 8 + k];
+      sig[i] = (R)a.sigma[i];
+      mdt[i] = (R)a.mu[i] * dt;
+      rsd[i] = R(1) / (sig[i] * sdt);
+      rLd[i] = R(1) / L[i][i];
+      Xb[i] = dpay * (R)a.weight[i];
+      gm[i] = R(0);
+      gs[i] = R(0);
+      Xp[i] = XN[i];
+    }
+#pragma unroll
+    for (int q = 0; q < NL; ++q) gL[q] = R(0);
+  }
+  __device__ __forceinline__ void step(const R (&x)[D]) {
+    basket_reverse_step<R, D>(x, Xp, Xb, gm, gs, gL, L, rLd, mdt, rsd, sig, sdt, dt);
+  }
+  // Gradient q of this path, q < 3 D + NL: d/dX0, d/dmu, d/dsigma, d/dL (q a constant after
+  // unrolling).
+  __device__ __forceinline__ double grad(int q) const {
+    if (q < D) return (double)Xb[q];
+    if (q < 2 * D) return (double)gm[q - D];
+    if (q < 3 * D) return (double)(gs[q - 2 * D] / sig[q - 2 * D]);
+    return (double)gL[q - 3 * D];
+  }
+};
+
+// One partial per block of each of the NQ per-path values get(q): a butterfly over the wave's
+// lanes, then the waves in order (the summation order of sde_block_reduce_store), eight
+// quantities at a time so that only eight f64 values per lane are live.
+template <int NQ, class G>
+__device__ __forceinline__ void basket_reduce_store(const G& get, double* __restrict__ out) {
+  __shared__ double red[16][8];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+  for (int q0 = 0; q0 < NQ; q0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) v[i] = q0 + i < NQ ? get(q0 + i) : 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off, 64);
+    }
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < 8; ++i) red[wave][i] = v[i];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < 8 && q0 + (int)threadIdx.x < NQ) {
+      double t = 0.0;
+      for (int w = 0; w < nw; ++w) t += red[w][threadIdx.x];
+      out[q0 + threadIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// MODE 0: price; 1: price and adjoint with checkpoints and recomputation; 2: the forward pass of
+// the tape form, which writes X_0 .. X_N to the tape and leaves the payoff, the reverse sweep and
+// every partial to sdesim_basket_reverse.
+template <typename R, int D, int SEG, int MODE>
 __global__ void __launch_bounds__(256) sdesim_basket(const BasketArgs a) {
-  constexpr int NL = BasketConst<D>::NL;
+  constexpr bool TAPE = MODE == 2;
   constexpr int NQ = BasketConst<D>::NQ;
   const long long S = a.paths;
   const int N = a.steps;
   const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const bool live = s < S;
   const long long sid = live ? s : 0;
+  const sde_u32 voff = live ? (sde_u32)(sid * (long long)sizeof(R)) : 0x80000000u;
 
-  R mu[D], sig[D], w[D], L[D][D];
+  R mu[D], sig[D], L[D][D];
 #pragma unroll
   for (int i = 0; i < D; ++i) {
     mu[i] = (R)a.mu[i];
     sig[i] = (R)a.sigma[i];
-    w[i] = (R)a.weight[i];
 #pragma unroll
     for (int k = 0; k < D; ++k) L[i][k] = (R)a.chol[i * 8 + k];
   }
   const R dt = (R)a.dt, sdt = (R)a.sdt;
-  const R K = (R)a.strike, width = (R)a.width;
 
   sde_u32 key[8];
   sde_seed_from_u64(a.seed + (sde_u64)sid, key);
@@ -118,10 +291,11 @@ __global__ void __launch_bounds__(256) sdesim_basket(const BasketArgs a) {
 #pragma unroll
   for (int i = 0; i < D; ++i) X[i] = (R)a.x0[i];
 
-  R* ck = (R*)a.checkpoints;
+  R* ck = (R*)a.states;
   // Forward pass.
   for (int n = 0; n < N; ++n) {
-    if (ADJ && (n % SEG) == 0 && live) {
+    if (TAPE) basket_tape_store<R, D>(a.states, n, S, voff, X);
+    if (MODE == 1 && (n % SEG) == 0 && live) {
       const long long slot = n / SEG;
 #pragma unroll
       for (int i = 0; i < D; ++i) ck[(slot * D + i) * S + sid] = X[i];
@@ -131,46 +305,23 @@ __global__ void __launch_bounds__(256) sdesim_basket(const BasketArgs a) {
 #pragma unroll
     for (int i = 0; i < D; ++i) X[i] = X[i] + X[i] * (mu[i] * dt + sig[i] * sdt * st.Z[i]);
   }
-  R B = R(0);
+  if (TAPE) {
+    basket_tape_store<R, D>(a.states, N, S, voff, X);
+    return;
+  }
+  R w[D];
 #pragma unroll
-  for (int i = 0; i < D; ++i) B = B + w[i] * X[i];
+  for (int i = 0; i < D; ++i) w[i] = (R)a.weight[i];
   R pay, dpay;
-  if (width > R(0)) {
-    const R x = (B - K) / width;
-    pay = width * basket_softplus<R>(x);
-    dpay = (R)(1.0 / (1.0 + exp(-(double)x)));
-  } else {
-    pay = B > K ? B - K : R(0);
-    dpay = B > K ? R(1) : R(0);
-  }
+  basket_payoff<R, D>(X, w, (R)a.strike, (R)a.width, pay, dpay);
 
-  double acc[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-  if (live) {
-    acc[0] = (double)pay;
-    acc[1] = (double)pay * (double)pay;
+  if (MODE == 0) {
+    basket_reduce_store<NQ>([&](int q) { return !live || q > 1 ? 0.0 : q == 0 ? (double)pay : (double)pay * (double)pay; },
+                            a.partials + (long long)blockIdx.x * NQ);
   }
-
-  if (ADJ) {
-    R Xb[D], gm[D], gs[D], gL[NL];
-#pragma unroll
-    for (int i = 0; i < D; ++i) { Xb[i] = dpay * w[i]; gm[i] = R(0); gs[i] = R(0); }
-#pragma unroll
-    for (int q = 0; q < NL; ++q) gL[q] = R(0);
-    // Per-asset constants of the reverse step: mu dt, 1 / (sigma sqrt(dt)), 1 / L_ii.
-    R mdt[D], rsd[D], rLd[D];
-#pragma unroll
-    for (int i = 0; i < D; ++i) {
-      mdt[i] = mu[i] * dt;
-      rsd[i] = R(1) / (sig[i] * sdt);
-      rLd[i] = R(1) / L[i][i];
-    }
-    R Xp[D];  // the state after the step being reversed: X_N first, then each state stepped back to
-#pragma unroll
-    for (int i = 0; i < D; ++i) Xp[i] = X[i];
-    const int nseg = (N + SEG - 1) / SEG;
-    for (int seg = nseg - 1; seg >= 0; --seg) {
+  if (MODE == 1) {
+    BasketReverse<R, D> rv(a, X, dpay);
+    for (int seg = (N + SEG - 1) / SEG - 1; seg >= 0; --seg) {
       const int n0 = seg * SEG;
       R Xs[SEG][D];
       R Xc[D];
@@ -191,52 +342,48 @@ __global__ void __launch_bounds__(256) sdesim_basket(const BasketArgs a) {
       // Step back through it.
 #pragma unroll
       for (int k = SEG - 1; k >= 0; --k) {
-        if (n0 + k < N) {
-          R Zb[D], Z[D], eps[D];
-#pragma unroll
-          for (int i = 0; i < D; ++i) {
-            const R xb = Xb[i], x = Xs[k][i], xp = Xp[i];
-            const R rx = basket_recip<R>(x);
-            const R d = (xp - x) - x * mdt[i];
-            gm[i] = gm[i] + xb * x * dt;
-            gs[i] = gs[i] + xb * d;
-            Zb[i] = xb * x * sig[i] * sdt;
-            Z[i] = d * rx * rsd[i];
-            Xb[i] = xb * (xp * rx);
-            Xp[i] = x;
-          }
-#pragma unroll
-          for (int i = 0; i < D; ++i) {
-            R r = Z[i];
-#pragma unroll
-            for (int kk = 0; kk < i; ++kk) r = r - L[i][kk] * eps[kk];
-            eps[i] = r * rLd[i];
-          }
-#pragma unroll
-          for (int i = 0; i < D; ++i) {
-#pragma unroll
-            for (int kk = 0; kk <= i; ++kk) {
-              const int q = i * (i + 1) / 2 + kk;
-              gL[q] = gL[q] + Zb[i] * eps[kk];
-            }
-          }
-        }
+        if (n0 + k < N) rv.step(Xs[k]);
       }
     }
-#pragma unroll
-    for (int i = 0; i < D; ++i) gs[i] = gs[i] / sig[i];
-    if (live) {
-#pragma unroll
-      for (int i = 0; i < D; ++i) {
-        acc[2 + i] = (double)Xb[i];
-        acc[2 + D + i] = (double)gm[i];
-        acc[2 + 2 * D + i] = (double)gs[i];
-      }
-#pragma unroll
-      for (int q = 0; q < NL; ++q) acc[2 + 3 * D + q] = (double)gL[q];
-    }
+    basket_reduce_store<NQ>(
+        [&](int q) { return !live ? 0.0 : q == 0 ? (double)pay : q == 1 ? (double)pay * (double)pay : rv.grad(q - 2); },
+        a.partials + (long long)blockIdx.x * NQ);
   }
-  sde_block_reduce_store<NQ>(acc, a.partials + (long long)blockIdx.x * NQ);
+}
+
+// Reverse sweep of the tape form: X_N from the tape gives the payoff, then the states X_{N-1} ..
+// X_0 are read back in two alternating register buffers, each requested one reverse step before
+// its use (the requests past step 0 are clamped to step 0 and unused).
+template <typename R, int D>
+__global__ void __launch_bounds__(256) sdesim_basket_reverse(const BasketArgs a) {
+  constexpr int NQ = BasketConst<D>::NQ;
+  const long long S = a.paths;
+  const int N = a.steps;
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = s < S;
+  const sde_u32 voff = live ? (sde_u32)(s * (long long)sizeof(R)) : 0x80000000u;
+
+  R X[D], w[D];
+  basket_tape_load<R, D>(a.states, N, S, voff, X);
+#pragma unroll
+  for (int i = 0; i < D; ++i) w[i] = (R)a.weight[i];
+  R pay, dpay;
+  basket_payoff<R, D>(X, w, (R)a.strike, (R)a.width, pay, dpay);
+
+  BasketReverse<R, D> rv(a, X, dpay);
+  R xa[D], xb[D];
+  basket_tape_load<R, D>(a.states, N - 1, S, voff, xa);
+  int n = N - 1;
+  for (; n >= 1; n -= 2) {
+    basket_tape_load<R, D>(a.states, n - 1, S, voff, xb);
+    rv.step(xa);  // step n
+    basket_tape_load<R, D>(a.states, n >= 2 ? n - 2 : 0, S, voff, xa);
+    rv.step(xb);  // step n - 1
+  }
+  if (n == 0) rv.step(xa);
+  basket_reduce_store<NQ>(
+      [&](int q) { return !live ? 0.0 : q == 0 ? (double)pay : q == 1 ? (double)pay * (double)pay : rv.grad(q - 2); },
+      a.partials + (long long)blockIdx.x * NQ);
 }
 
 // Sum of per-block partials per quantity (one block per quantity, Neumaier-compensated).
@@ -269,21 +416,28 @@ __global__ void __launch_bounds__(256) sdesim_sum_partials(const double* __restr
 
 }  // namespace
 
-hipError_t sdesim_launch_basket(int f32, int assets, int adjoint, int grid, const BasketArgs& a, int nq,
+// mode: 0 price, 1 adjoint with checkpoints (a.states holds [slots][D][paths], one per 19 steps),
+// 2 adjoint with the tape (a.states holds [steps + 1][D][paths]; a forward and a reverse kernel).
+hipError_t sdesim_launch_basket(int f32, int assets, int mode, int grid, const BasketArgs& a, int nq,
                                 double* sums, hipStream_t stream) {
-#define SDE_BASKET(R, D, ADJ)                                                                              \
-  hipLaunchKernelGGL((sdesim_basket<R, D, 19, ADJ>), dim3(grid), dim3(256), 0, stream, a);                    \
+#define SDE_BASKET(R, D, MODE)                                                                             \
+  hipLaunchKernelGGL((sdesim_basket<R, D, 19, MODE>), dim3(grid), dim3(256), 0, stream, a);                   \
+  if (MODE == 2) hipLaunchKernelGGL((sdesim_basket_reverse<R, D>), dim3(grid), dim3(256), 0, stream, a);     \
   break;
   bool ok = true;
-  switch ((f32 ? 100 : 0) + assets * 2 + (adjoint ? 1 : 0)) {
-    case 100 + 16 + 1: SDE_BASKET(float, 8, true)
-    case 100 + 16 + 0: SDE_BASKET(float, 8, false)
-    case 100 + 2 + 1: SDE_BASKET(float, 1, true)
-    case 100 + 2 + 0: SDE_BASKET(float, 1, false)
-    case 16 + 1: SDE_BASKET(double, 8, true)
-    case 16 + 0: SDE_BASKET(double, 8, false)
-    case 2 + 1: SDE_BASKET(double, 1, true)
-    case 2 + 0: SDE_BASKET(double, 1, false)
+  switch ((f32 ? 100 : 0) + assets * 3 + mode) {
+    case 100 + 24 + 2: SDE_BASKET(float, 8, 2)
+    case 100 + 24 + 1: SDE_BASKET(float, 8, 1)
+    case 100 + 24 + 0: SDE_BASKET(float, 8, 0)
+    case 100 + 3 + 2: SDE_BASKET(float, 1, 2)
+    case 100 + 3 + 1: SDE_BASKET(float, 1, 1)
+    case 100 + 3 + 0: SDE_BASKET(float, 1, 0)
+    case 24 + 2: SDE_BASKET(double, 8, 2)
+    case 24 + 1: SDE_BASKET(double, 8, 1)
+    case 24 + 0: SDE_BASKET(double, 8, 0)
+    case 3 + 2: SDE_BASKET(double, 1, 2)
+    case 3 + 1: SDE_BASKET(double, 1, 1)
+    case 3 + 0: SDE_BASKET(double, 1, 0)
     default: ok = false;
   }
 #undef SDE_BASKET

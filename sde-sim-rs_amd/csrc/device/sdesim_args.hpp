@@ -79,7 +79,8 @@ struct BasketArgs {
   double dt, sdt, strike, width;
   sde_u64 seed;               // path s uses seed + s
   long long paths;
-  void* checkpoints;          // [slots][D][paths] real
+  void* states;               // adjoint storage: the tape [steps][D][paths] real, or checkpoints
+                              // [slots][D][paths] real (one slot per 19 steps)
   double* partials;           // [gridDim.x][NQ]
   int steps;
   int pad_;

@@ -13,7 +13,7 @@
 #include "../../include/sdesim.h"
 #include "device/sdesim_args.hpp"
 
-hipError_t sdesim_launch_basket(int f32, int assets, int adjoint, int grid, const BasketArgs& a, int nq,
+hipError_t sdesim_launch_basket(int f32, int assets, int mode, int grid, const BasketArgs& a, int nq,
                                 double* sums, hipStream_t stream);
 int sdesim_set_error(int code, const std::string& msg);
 
@@ -45,11 +45,12 @@ struct sdesim_basket {
   int D = 8;          // compiled width (1 or 8)
   int nq = 0;
   long long grid = 0;
-  void* ckpt = nullptr;
+  int storage = SDESIM_STORAGE_CHECKPOINTS;  // resolved at creation: TAPE or CHECKPOINTS
+  void* states = nullptr;                   // the tape or the checkpoints
   double* partials = nullptr;
   double* sums = nullptr;
   ~sdesim_basket() {
-    if (ckpt) (void)hipFree(ckpt);
+    if (states) (void)hipFree(states);
     if (partials) (void)hipFree(partials);
     if (sums) (void)hipFree(sums);
   }
@@ -66,6 +67,8 @@ int sdesim_basket_create(const sdesim_basket_spec* spec, sdesim_basket** out) {
     if (s.paths < 1) throw BFail{SDESIM_EVALUE, "paths must be >= 1"};
     if (!(s.horizon > 0.0)) throw BFail{SDESIM_EVALUE, "horizon must be > 0"};
     if (!(s.width >= 0.0)) throw BFail{SDESIM_EVALUE, "width must be >= 0"};
+    if (s.storage < SDESIM_STORAGE_AUTO || s.storage > SDESIM_STORAGE_CHECKPOINTS)
+      throw BFail{SDESIM_EVALUE, "storage must be SDESIM_STORAGE_AUTO, _TAPE or _CHECKPOINTS"};
     std::unique_ptr<sdesim_basket> b(new sdesim_basket());
     b->spec = s;
     b->D = s.assets == 1 ? 1 : 8;
@@ -75,10 +78,24 @@ int sdesim_basket_create(const sdesim_basket_spec* spec, sdesim_basket** out) {
     const size_t rb = s.precision == SDESIM_F32 ? 4 : 8;
     const long long slots = (s.steps + kSeg - 1) / kSeg;
     const size_t ck = (size_t)slots * b->D * (size_t)s.paths * rb;
+    const size_t step_bytes = (size_t)b->D * (size_t)s.paths * rb;  // one step of the tape
+    const size_t tape = step_bytes * ((size_t)s.steps + 1);  // X_0 .. X_N
     size_t free_b = 0, total_b = 0;
     bk(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    if (ck + (64u << 20) > free_b) throw BFail{SDESIM_ERUNTIME, "checkpoints do not fit in free device memory"};
-    bk(hipMalloc(&b->ckpt, ck), "hipMalloc(checkpoints)");
+    const bool tape_addressable = step_bytes < (1ull << 31);  // one buffer resource per step
+    if (s.storage == SDESIM_STORAGE_TAPE) {
+      if (!tape_addressable) throw BFail{SDESIM_EVALUE, "the tape needs assets * paths * size < 2^31 bytes per step"};
+      if (tape + (64u << 20) > free_b) throw BFail{SDESIM_ERUNTIME, "the tape does not fit in free device memory"};
+      b->storage = SDESIM_STORAGE_TAPE;
+    } else if (s.storage == SDESIM_STORAGE_AUTO && tape_addressable && tape + (64u << 20) <= free_b / 2) {
+      b->storage = SDESIM_STORAGE_TAPE;
+    }
+    if (b->storage == SDESIM_STORAGE_TAPE) {
+      bk(hipMalloc(&b->states, tape), "hipMalloc(tape)");
+    } else {
+      if (ck + (64u << 20) > free_b) throw BFail{SDESIM_ERUNTIME, "checkpoints do not fit in free device memory"};
+      bk(hipMalloc(&b->states, ck), "hipMalloc(checkpoints)");
+    }
     bk(hipMalloc(&b->partials, sizeof(double) * b->nq * b->grid), "hipMalloc(partials)");
     bk(hipMalloc(&b->sums, sizeof(double) * b->nq), "hipMalloc(sums)");
     *out = b.release();
@@ -86,6 +103,8 @@ int sdesim_basket_create(const sdesim_basket_spec* spec, sdesim_basket** out) {
 }
 
 void sdesim_basket_free(sdesim_basket* b) { delete b; }
+
+int32_t sdesim_basket_storage(const sdesim_basket* b) { return b ? b->storage : -1; }
 
 int sdesim_basket_eval(sdesim_basket* b, const double* x0, const double* mu, const double* sigma,
                        const double* weight, const double* chol, int32_t adjoint, double* out, void* stream) {
@@ -110,10 +129,11 @@ int sdesim_basket_eval(sdesim_basket* b, const double* x0, const double* mu, con
     a.seed = s.seed;
     a.paths = s.paths;
     a.steps = s.steps;
-    a.checkpoints = b->ckpt;
+    a.states = b->states;
     a.partials = b->partials;
     hipStream_t st = (hipStream_t)stream;
-    bk(sdesim_launch_basket(s.precision == SDESIM_F32 ? 1 : 0, D, adjoint ? 1 : 0, (int)b->grid, a, b->nq, b->sums, st),
+    const int mode = !adjoint ? 0 : b->storage == SDESIM_STORAGE_TAPE ? 2 : 1;
+    bk(sdesim_launch_basket(s.precision == SDESIM_F32 ? 1 : 0, D, mode, (int)b->grid, a, b->nq, b->sums, st),
        "basket launch");
     std::vector<double> sums(b->nq);
     bk(hipMemcpyAsync(sums.data(), b->sums, sizeof(double) * b->nq, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");

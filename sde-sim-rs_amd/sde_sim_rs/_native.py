@@ -16,6 +16,7 @@ SCRAMBLE_SHIFT, SCRAMBLE_XOR = 0, 1
 F64, F32 = 0, 1
 COMPAT_REFERENCE, COMPAT_TEXTBOOK = 0, 1
 OUT_FILTRATION, OUT_TERMINAL = 1, 2
+STORAGE_AUTO, STORAGE_TAPE, STORAGE_CHECKPOINTS = 0, 1, 2
 
 
 class Options(ctypes.Structure):
@@ -58,7 +59,7 @@ class BasketSpec(ctypes.Structure):
         ("width", ctypes.c_double),
         ("seed", ctypes.c_uint64),
         ("precision", ctypes.c_int32),
-        ("pad_", ctypes.c_int32),
+        ("storage", ctypes.c_int32),
     ]
 
 
@@ -127,6 +128,7 @@ SIGNATURES = {
                                          ctypes.POINTER(ctypes.c_int64)]),
     "sdesim_basket_create": (ctypes.c_int, [ctypes.POINTER(BasketSpec), ctypes.POINTER(ctypes.c_void_p)]),
     "sdesim_basket_free": (None, [ctypes.c_void_p]),
+    "sdesim_basket_storage": (ctypes.c_int32, [ctypes.c_void_p]),
     "sdesim_basket_eval": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_double)] * 5 +
                            [ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
     "sdesim_lmm_create": (ctypes.c_int, [ctypes.POINTER(LmmSpec), ctypes.POINTER(ctypes.c_void_p)]),

@@ -60,10 +60,21 @@ def basket_dsl(x0, mu, sigma, rho):
     return eqs, init
 
 
-class Basket:
-    """Price and adjoint of a weighted basket call on D <= 8 correlated GBM assets."""
+_STORAGE = {"auto": nat.STORAGE_AUTO, "tape": nat.STORAGE_TAPE, "checkpoints": nat.STORAGE_CHECKPOINTS}
 
-    def __init__(self, assets, steps, paths, horizon=1.0, strike=100.0, width=0.5, seed=0, precision="f32"):
+
+class Basket:
+    """Price and adjoint of a weighted basket call on D <= 8 correlated GBM assets.
+
+    `storage` picks how the adjoint keeps the forward states: "tape" (every state in device
+    memory, steps * 8 * paths values for D > 1), "checkpoints" (every 19th state, the rest
+    recomputed during the reverse sweep) or "auto" (the tape when it fits in half of the free
+    device memory). `self.storage` is the form in use."""
+
+    def __init__(self, assets, steps, paths, horizon=1.0, strike=100.0, width=0.5, seed=0, precision="f32",
+                 storage="auto"):
+        if storage not in _STORAGE:
+            raise ValueError("storage must be 'auto', 'tape' or 'checkpoints'")
         spec = nat.BasketSpec()
         spec.assets = assets
         spec.steps = steps
@@ -73,12 +84,14 @@ class Basket:
         spec.width = width
         spec.seed = int(seed) & (2 ** 64 - 1)
         spec.precision = nat.F32 if precision in ("f32", "float32") else nat.F64
+        spec.storage = _STORAGE[storage]
         self.assets = assets
         self.steps, self.paths, self.horizon, self.strike, self.width = steps, paths, horizon, strike, width
         self.precision = precision
         h = ctypes.c_void_p()
         nat.check(nat.lib.sdesim_basket_create(ctypes.byref(spec), ctypes.byref(h)))
         self._h = h
+        self.storage = "tape" if nat.lib.sdesim_basket_storage(h) == nat.STORAGE_TAPE else "checkpoints"
 
     def evaluate(self, x0, mu, sigma, rho, weights=None, adjoint=True, stream=None):
         D = self.assets

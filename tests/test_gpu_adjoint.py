@@ -132,3 +132,26 @@ def test_torch_autograd_function():
     r = b.evaluate(c["x0"], c["mu"], c["sigma"], c["rho"])
     np.testing.assert_allclose(x0.grad.numpy(), 2 * r["x0"], rtol=1e-12)
     np.testing.assert_allclose(rho.grad.numpy(), 2 * r["rho"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("precision,D", [("f64", 8), ("f32", 8), ("f64", 1)])
+def test_adjoint_tape_equals_checkpoints(precision, D):
+    """The two state storages of the adjoint (every state on a tape; checkpoints every 19 steps
+    with recomputation) reverse the same paths: price and gradients agree to rounding. 2^12 + 37
+    paths (a partial last block), 45 steps (two full checkpoint segments and a partial one)."""
+    from sde_sim_rs.adjoint import Basket
+    c = _cfg()
+    x0, mu, sg, rho = c["x0"][:D], c["mu"][:D], c["sigma"][:D], c["rho"][:D, :D]
+    S, N = (1 << 12) + 37, 45
+    print("case: basket %s D=%d tape vs checkpoints S=%d N=%d" % (precision, D, S, N), flush=True)
+    out = {}
+    for storage in ("tape", "checkpoints"):
+        b = Basket(D, N, S, strike=100.0, width=0.5, seed=11, precision=precision, storage=storage)
+        assert b.storage == storage
+        out[storage] = b.evaluate(x0, mu, sg, rho, adjoint=True)
+    rtol = 1e-12 if precision == "f64" else 1e-5
+    t, k = out["tape"], out["checkpoints"]
+    assert abs(t["price"] - k["price"]) <= rtol * abs(k["price"])
+    for name in ("x0", "mu", "sigma", "chol"):
+        scale = np.max(np.abs(k[name]))
+        np.testing.assert_allclose(t[name], k[name], rtol=0, atol=rtol * scale, err_msg=name)
