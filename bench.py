@@ -128,8 +128,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true", help="run a 1 GiB device copy first (HBM counter calibration)")
     ap.add_argument("--only", default="", help="comma list of legs to run: f32,f64,term,batch,adjoint,lmm")
-    ap.add_argument("--basket-storage", default="auto", choices=("auto", "tape", "checkpoints"),
-                    help="state storage of the basket adjoint")
+    ap.add_argument("--adjoint-storage", default="auto", choices=("auto", "tape", "checkpoints"),
+                    help="state storage of the basket and forward-rate adjoints")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: paths are independent and the multi-GPU form "
@@ -207,7 +207,7 @@ def main():
         c = basket_config(seed=3)
         AS, AN = 1 << 22, 365
         bk = Basket(8, AN, AS, horizon=1.0, strike=100.0, width=0.5, seed=3, precision="f32",
-                    storage=args.basket_storage)
+                    storage=args.adjoint_storage)
         times = {}
         for mode in (False, True):
             for _ in range(max(1, args.warmup)):
@@ -230,7 +230,8 @@ def main():
         from sde_sim_rs.lmm import ForwardRateModel, forward_rate_config
         c = forward_rate_config(seed=5)
         LS, LN = 1 << 20, 160
-        fm = ForwardRateModel(steps=LN, paths=LS, expiry_index=4, strike=0.03, scramble_seed=5, precision="f32")
+        fm = ForwardRateModel(steps=LN, paths=LS, expiry_index=4, strike=0.03, scramble_seed=5, precision="f32",
+                              storage=args.adjoint_storage)
         times = {}
         for mode in (False, True):
             for _ in range(max(1, args.warmup)):
@@ -243,7 +244,8 @@ def main():
         lmm = {"workload": "40 forwards, 3 factors, 1y x 9y receiver swaption K=0.03, 160 log-Euler steps, "
                            "2^20 paths, Sobol-XOR dim 480, f32",
                "sensitivities": 40, "adjoint_ms": times[True], "price_ms": times[False],
-               "forward_steps_per_s_price": LS * LN * 40 / (times[False] * 1e-3), "price": r["price"]}
+               "forward_steps_per_s_price": LS * LN * 40 / (times[False] * 1e-3), "price": r["price"],
+               "storage": fm.storage}
         del fm
 
     path_steps = S * N

@@ -205,10 +205,11 @@ typedef struct sdesim_basket_spec {
   int32_t storage;    /* adjoint state storage: SDESIM_STORAGE_AUTO, _TAPE or _CHECKPOINTS */
 } sdesim_basket_spec;
 
-/* Adjoint state storage. TAPE keeps every state in device memory (steps * assets * paths values
- * in the path precision, one write and one read each; needs assets * paths * size < 2^31).
- * CHECKPOINTS keeps every 19th state and recomputes the rest during the reverse sweep (one more
- * forward pass of arithmetic, 19 times less memory). AUTO takes the tape when it fits in half of
+/* Adjoint state storage (basket and forward-rate model). TAPE keeps every state in device memory
+ * ((steps + 1) * state size * paths values in the path precision, one write and one read each;
+ * one step's states must stay below 2^31 bytes). CHECKPOINTS keeps every 19th (basket) or 16th
+ * (forward-rate model) state and recomputes the rest during the reverse sweep (one more forward
+ * pass of arithmetic, for that factor less memory). AUTO takes the tape when it fits in half of
  * the free device memory at creation, otherwise checkpoints. */
 #define SDESIM_STORAGE_AUTO 0
 #define SDESIM_STORAGE_TAPE 1
@@ -246,10 +247,15 @@ typedef struct sdesim_lmm_spec {
   double tenor;
   double strike;
   uint64_t scramble_seed;
+  int32_t storage;           /* adjoint state storage: SDESIM_STORAGE_AUTO, _TAPE or _CHECKPOINTS
+                              * (checkpoints every 16 steps, each segment recomputed once) */
+  int32_t pad_;
 } sdesim_lmm_spec;
 
 int sdesim_lmm_create(const sdesim_lmm_spec* spec, sdesim_lmm** out);
 void sdesim_lmm_free(sdesim_lmm* m);
+/* The storage the handle uses: SDESIM_STORAGE_TAPE or SDESIM_STORAGE_CHECKPOINTS. */
+int32_t sdesim_lmm_storage(const sdesim_lmm* m);
 /* out[0] = price, out[1] = sample variance of the discounted payoff, and with adjoint != 0
  * out[2 + i] = d price / d L_i(0), i = 0..39. Synchronises the stream. */
 int sdesim_lmm_eval(sdesim_lmm* m, const double* L0, const double* vol, const double* loadings, int32_t adjoint,

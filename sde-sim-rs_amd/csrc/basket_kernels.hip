@@ -28,6 +28,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device/sdesim_device.hpp"
+#include "adjoint.hpp"
 
 namespace {
 
@@ -127,45 +128,6 @@ __device__ __forceinline__ void basket_reverse_step(const R (&x)[D], R (&xp)[D],
   }
 }
 
-// The tape's states of step n: rows [n][asset][path] under one buffer resource (D * S * sizeof(R)
-// bytes, below 2^31, checked by the host); the path's byte offset is the vector offset, the
-// asset's row the scalar offset. Lanes past the last path use vector offset 2^31, beyond the
-// resource: their stores are dropped and their loads return 0 (values that are never reduced).
-template <typename R, int D>
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t basket_tape_rsrc(void* tape, int n, long long S) {
-  // n is wave-uniform; readfirstlane keeps the resource in scalar registers even where the index
-  // was formed by a vector instruction (a clamped subtraction), which would otherwise make the
-  // compiler wrap every access in a loop over distinct resources.
-  n = __builtin_amdgcn_readfirstlane(n);
-  const long long step_bytes = (long long)D * S * (long long)sizeof(R);
-  return __builtin_amdgcn_make_buffer_rsrc((void*)((char*)tape + (long long)n * step_bytes), (short)0, (int)step_bytes,
-                                           0x00020000);
-}
-template <typename R, int D>
-__device__ __forceinline__ void basket_tape_store(void* tape, int n, long long S, sde_u32 voff, const R (&X)[D]) {
-  const __amdgpu_buffer_rsrc_t rs = basket_tape_rsrc<R, D>(tape, n, S);
-#pragma unroll
-  for (int i = 0; i < D; ++i) {
-    const int soff = (int)((long long)i * S * (long long)sizeof(R));
-    if constexpr (sizeof(R) == 4)
-      __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(sde_u32, (float)X[i]), rs, (int)voff, soff, 2);
-    else
-      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(SdeU32x2, (double)X[i]), rs, (int)voff, soff, 2);
-  }
-}
-template <typename R, int D>
-__device__ __forceinline__ void basket_tape_load(void* tape, int n, long long S, sde_u32 voff, R (&X)[D]) {
-  const __amdgpu_buffer_rsrc_t rs = basket_tape_rsrc<R, D>(tape, n, S);
-#pragma unroll
-  for (int i = 0; i < D; ++i) {
-    const int soff = (int)((long long)i * S * (long long)sizeof(R));
-    if constexpr (sizeof(R) == 4)
-      X[i] = __builtin_bit_cast(float, (sde_u32)__builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, soff, 2));
-    else
-      X[i] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, soff, 2));
-  }
-}
-
 template <typename R>
 __device__ __forceinline__ R basket_softplus(R x) {
   // log(1 + e^x) without overflow: max(x, 0) + log1p(exp(-|x|))
@@ -229,38 +191,6 @@ struct BasketReverse {
   }
 };
 
-// One partial per block of each of the NQ per-path values get(q): a butterfly over the wave's
-// lanes, then the waves in order (the summation order of sde_block_reduce_store), eight
-// quantities at a time so that only eight f64 values per lane are live.
-template <int NQ, class G>
-__device__ __forceinline__ void basket_reduce_store(const G& get, double* __restrict__ out) {
-  __shared__ double red[16][8];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int nw = (blockDim.x + 63) >> 6;
-#pragma unroll
-  for (int q0 = 0; q0 < NQ; q0 += 8) {
-    double v[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) v[i] = q0 + i < NQ ? get(q0 + i) : 0.0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) v[i] += __shfl_xor(v[i], off, 64);
-    }
-    if (lane == 0) {
-#pragma unroll
-      for (int i = 0; i < 8; ++i) red[wave][i] = v[i];
-    }
-    __syncthreads();
-    if ((int)threadIdx.x < 8 && q0 + (int)threadIdx.x < NQ) {
-      double t = 0.0;
-      for (int w = 0; w < nw; ++w) t += red[w][threadIdx.x];
-      out[q0 + threadIdx.x] = t;
-    }
-    __syncthreads();
-  }
-}
-
 // MODE 0: price; 1: price and adjoint with checkpoints and recomputation; 2: the forward pass of
 // the tape form, which writes X_0 .. X_N to the tape and leaves the payoff, the reverse sweep and
 // every partial to sdesim_basket_reverse.
@@ -294,7 +224,7 @@ __global__ void __launch_bounds__(256) sdesim_basket(const BasketArgs a) {
   R* ck = (R*)a.states;
   // Forward pass.
   for (int n = 0; n < N; ++n) {
-    if (TAPE) basket_tape_store<R, D>(a.states, n, S, voff, X);
+    if (TAPE) sde_tape_store<R, D>(a.states, n, S, voff, X);
     if (MODE == 1 && (n % SEG) == 0 && live) {
       const long long slot = n / SEG;
 #pragma unroll
@@ -306,7 +236,7 @@ __global__ void __launch_bounds__(256) sdesim_basket(const BasketArgs a) {
     for (int i = 0; i < D; ++i) X[i] = X[i] + X[i] * (mu[i] * dt + sig[i] * sdt * st.Z[i]);
   }
   if (TAPE) {
-    basket_tape_store<R, D>(a.states, N, S, voff, X);
+    sde_tape_store<R, D>(a.states, N, S, voff, X);
     return;
   }
   R w[D];
@@ -316,7 +246,7 @@ __global__ void __launch_bounds__(256) sdesim_basket(const BasketArgs a) {
   basket_payoff<R, D>(X, w, (R)a.strike, (R)a.width, pay, dpay);
 
   if (MODE == 0) {
-    basket_reduce_store<NQ>([&](int q) { return !live || q > 1 ? 0.0 : q == 0 ? (double)pay : (double)pay * (double)pay; },
+    sde_reduce_store_chunked<NQ>([&](int q) { return !live || q > 1 ? 0.0 : q == 0 ? (double)pay : (double)pay * (double)pay; },
                             a.partials + (long long)blockIdx.x * NQ);
   }
   if (MODE == 1) {
@@ -345,7 +275,7 @@ __global__ void __launch_bounds__(256) sdesim_basket(const BasketArgs a) {
         if (n0 + k < N) rv.step(Xs[k]);
       }
     }
-    basket_reduce_store<NQ>(
+    sde_reduce_store_chunked<NQ>(
         [&](int q) { return !live ? 0.0 : q == 0 ? (double)pay : q == 1 ? (double)pay * (double)pay : rv.grad(q - 2); },
         a.partials + (long long)blockIdx.x * NQ);
   }
@@ -364,7 +294,7 @@ __global__ void __launch_bounds__(256) sdesim_basket_reverse(const BasketArgs a)
   const sde_u32 voff = live ? (sde_u32)(s * (long long)sizeof(R)) : 0x80000000u;
 
   R X[D], w[D];
-  basket_tape_load<R, D>(a.states, N, S, voff, X);
+  sde_tape_load<R, D>(a.states, N, S, voff, X);
 #pragma unroll
   for (int i = 0; i < D; ++i) w[i] = (R)a.weight[i];
   R pay, dpay;
@@ -372,16 +302,16 @@ __global__ void __launch_bounds__(256) sdesim_basket_reverse(const BasketArgs a)
 
   BasketReverse<R, D> rv(a, X, dpay);
   R xa[D], xb[D];
-  basket_tape_load<R, D>(a.states, N - 1, S, voff, xa);
+  sde_tape_load<R, D>(a.states, N - 1, S, voff, xa);
   int n = N - 1;
   for (; n >= 1; n -= 2) {
-    basket_tape_load<R, D>(a.states, n - 1, S, voff, xb);
+    sde_tape_load<R, D>(a.states, n - 1, S, voff, xb);
     rv.step(xa);  // step n
-    basket_tape_load<R, D>(a.states, n >= 2 ? n - 2 : 0, S, voff, xa);
+    sde_tape_load<R, D>(a.states, n >= 2 ? n - 2 : 0, S, voff, xa);
     rv.step(xb);  // step n - 1
   }
   if (n == 0) rv.step(xa);
-  basket_reduce_store<NQ>(
+  sde_reduce_store_chunked<NQ>(
       [&](int q) { return !live ? 0.0 : q == 0 ? (double)pay : q == 1 ? (double)pay * (double)pay : rv.grad(q - 2); },
       a.partials + (long long)blockIdx.x * NQ);
 }

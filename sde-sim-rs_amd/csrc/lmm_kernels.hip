@@ -14,16 +14,21 @@
 // fixed rate K: V = max(K A - (1 - P_{NF-1}), 0) / N, P_j = prod_{k=e..j} 1 / (1 + tau L_k),
 // A = tau sum_{j=e..NF-1} P_j, numeraire N = prod_{k<e} (1 + tau L_k).
 //
-// Adjoint: the forward pass stores L every SEG steps (checkpoints in HBM, [slot][i][path]). The
-// reverse sweep takes the segments last to first: it recomputes a segment's states once from its
-// checkpoint into the segment tape ([k][i][path] in HBM, coalesced, rewritten per segment) and
-// then steps back through them. For one step, with a_j = tau sig_j L_j / (1 + tau L_j) and
+// Adjoint: the reverse sweep needs every state L_n. Two storage forms:
+//  * state tape (MODE 2): the forward pass writes L_0 .. L_N to HBM ([n][i][path], 27 GB for
+//    161 states x 40 forwards x 2^20 paths in f32) and a second kernel reads them back;
+//  * checkpoints (MODE 1): the forward pass stores L every SEG steps ([slot][i][path]); the
+//    reverse sweep takes the segments last to first, recomputes a segment's states once from its
+//    checkpoint into the segment tape ([k][i][path], rewritten per segment) and then steps back
+//    through them.
+// For one step, with a_j = tau sig_j L_j / (1 + tau L_j) and
 // E_i = L'_i / L_i (read as the ratio of the stored states, so the reverse step needs no exp):
 //   Cbar_f(i) = Lbar'_i L'_i dt sig_i B_if,   G_f(j) = sum_{i >= j} Cbar_f(i)  (a suffix sum),
 //   Lbar_j = Lbar'_j E_j + (sum_f B_jf G_f(j)) tau sig_j / (1 + tau L_j)^2.
 #include <hip/hip_runtime.h>
 
 #include "device/sdesim_device.hpp"
+#include "adjoint.hpp"
 
 namespace {
 
@@ -31,11 +36,6 @@ constexpr int NF = 40;
 constexpr int F = 3;
 constexpr int SEG = 16;  // checkpoint interval (~sqrt of the 160-step workload)
 constexpr int NSL = 3;  // Sobol table slices: point indices below 2^22
-
-template <typename R>
-struct LmmStep {
-  R eps[F];
-};
 
 // exp of an exponent given in the parameter rows' units: base 2 for f32 (the rows carry the
 // log2(e) factor, so this is one v_exp_f32), natural for f64.
@@ -117,44 +117,77 @@ __device__ __forceinline__ void lmm_step(const LmmPar<R>& tab, R (&L)[NF], const
   }
 }
 
+// Adjoint of one forward's part of lmm_step, forwards taken from the last down: Li and Lpi are
+// L_i before and after the step, Lbi the adjoint of Lp_i on entry and of L_i on exit, G the
+// running sums over the forwards above. Forwards are positive, so E_i = Lp_i / L_i.
+template <typename R>
+__device__ __forceinline__ void lmm_forward_adjoint(const LmmParRow<R>& pr, R Li, R Lpi, R& Lbi, R (&G)[F], R dt,
+                                                    R tau) {
+  const R sig = pr.sig, b0 = pr.b0, b1 = pr.b1, b2 = pr.b2;
+  const R db = Lbi * Lpi * dt;
+  G[0] = G[0] + db * sig * b0;
+  G[1] = G[1] + db * sig * b1;
+  G[2] = G[2] + db * sig * b2;
+  const R ab = b0 * G[0] + b1 * G[1] + b2 * G[2];
+  const R rop = lmm_recip<R>(R(1) + tau * Li);
+  Lbi = Lbi * (Lpi * lmm_recip<R>(Li)) + ab * pr.ts * (rop * rop);
+}
+
 // Adjoint of lmm_step: L is the state before the step and Lp the state after it; on entry Lb is
-// the adjoint of Lp, on exit the adjoint of L. Forwards are positive, so E_i = Lp_i / L_i.
+// the adjoint of Lp, on exit the adjoint of L.
 template <typename R>
 __device__ __forceinline__ void lmm_step_adjoint(const LmmPar<R>& tab, const R (&L)[NF], const R (&Lp)[NF],
                                                  R (&Lb)[NF], int eta, R dt, R tau) {
   R G[F] = {R(0), R(0), R(0)};
 #pragma unroll
   for (int i = NF - 1; i >= 0; --i) {
-    if (i >= eta) {
-      const LmmParRow<R> pr = tab.row(i);
-      const R sig = pr.sig, b0 = pr.b0, b1 = pr.b1, b2 = pr.b2;
-      const R db = Lb[i] * Lp[i] * dt;
-      G[0] = G[0] + db * sig * b0;
-      G[1] = G[1] + db * sig * b1;
-      G[2] = G[2] + db * sig * b2;
-      const R ab = b0 * G[0] + b1 * G[1] + b2 * G[2];
-      const R rop = lmm_recip<R>(R(1) + tau * L[i]);
-      Lb[i] = Lb[i] * (Lp[i] * lmm_recip<R>(L[i])) + ab * pr.ts * (rop * rop);
-    }
+    if (i >= eta) lmm_forward_adjoint<R>(tab.row(i), L[i], Lp[i], Lb[i], G, dt, tau);
   }
 }
 
-// The f32 adjoint is held to 168 VGPRs (3 waves per SIMD instead of 2 at its natural 172, for a
-// few spilled registers): the extra wave hides the tape and LDS latency of the reverse sweep.
-template <typename R, bool ADJ>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ADJ && sizeof(R) == 4 ? 3 : 1)))
-sdesim_lmm(const LmmArgs a) {
-  constexpr int NQ = 2 + NF;
-  const long long S = a.paths;
-  const int N = a.steps;
-  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool live = s < S;
-  const long long sid = live ? s : 0;
-  const R dt = (R)a.dt, sdt = (R)a.sdt, tau = (R)a.tau, K = (R)a.strike;
-  const int e = a.expiry_index;
+// Payoff at the final forwards L (discounted by the spot numeraire) and, with GRAD, its
+// derivatives dV/dL_k in Lb: the numeraire part for k < e, the swap part for k >= e (only when
+// in the money).
+template <typename R, bool GRAD>
+__device__ __forceinline__ R lmm_payoff(const R (&L)[NF], int e, R tau, R K, R (&Lb)[NF]) {
+  R numer = R(1);
+#pragma unroll
+  for (int k = 0; k < NF; ++k)
+    if (k < e) numer = numer * (R(1) + tau * L[k]);
+  R P = R(1), A = R(0);
+  R Pj[NF];
+#pragma unroll
+  for (int j = 0; j < NF; ++j) {
+    Pj[j] = R(0);
+    if (j >= e) {
+      P = P / (R(1) + tau * L[j]);
+      Pj[j] = P;
+      A = A + tau * P;
+    }
+  }
+  const R U = K * A - (R(1) - P);
+  const R V = U > R(0) ? U / numer : R(0);
+  if (GRAD) {
+    R tail = R(0);  // sum_{j >= k} P_j, built from the end
+#pragma unroll
+    for (int k = NF - 1; k >= 0; --k) {
+      if (k >= e) tail = tail + Pj[k];
+      const R op = R(1) + tau * L[k];
+      if (k < e) {
+        Lb[k] = -V * tau / op;
+      } else {
+        Lb[k] = U > R(0) ? (-K * tau * tau * tail - tau * P) / (op * numer) : R(0);
+      }
+    }
+  }
+  return V;
+}
 
-  __shared__ LmmParRow<R> par_rows[NF];
+// Model parameters staged into LDS by the first NF threads of the workgroup.
+template <typename R>
+__device__ __forceinline__ void lmm_stage_params(const LmmArgs& a, LmmParRow<R>* par_rows) {
   if (threadIdx.x < NF) {
+    const R dt = (R)a.dt, sdt = (R)a.sdt, tau = (R)a.tau;
     const R* par = sizeof(R) == 4 ? (const R*)(const void*)a.parf : (const R*)(const void*)a.pard;
     const int i = (int)threadIdx.x;
     const R sig = par[i];
@@ -163,6 +196,28 @@ sdesim_lmm(const LmmArgs a) {
                                sig * dt * un, sig * sdt * un, R(-0.5) * sig * sig * dt * un};
   }
   __syncthreads();
+}
+
+// MODE 0: price; 1: price and adjoint with checkpoints, the reverse sweep recomputing each
+// segment into the segment tape; 2: the forward pass of the tape form, which writes L_0 .. L_N to
+// the state tape and leaves the payoff, the reverse sweep and every partial to
+// sdesim_lmm_reverse. The f32 checkpoint adjoint is held to 3 waves per SIMD (168 VGPRs): the
+// extra wave hides the tape and LDS latency of its reverse sweep.
+template <typename R, int MODE>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE == 1 && sizeof(R) == 4 ? 3 : 1)))
+sdesim_lmm(const LmmArgs a) {
+  constexpr int NQ = 2 + NF;
+  const long long S = a.paths;
+  const int N = a.steps;
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = s < S;
+  const long long sid = live ? s : 0;
+  const sde_u32 voff = live ? (sde_u32)(sid * (long long)sizeof(R)) : 0x80000000u;
+  const R dt = (R)a.dt, sdt = (R)a.sdt, tau = (R)a.tau, K = (R)a.strike;
+  const int e = a.expiry_index;
+
+  __shared__ LmmParRow<R> par_rows[NF];
+  lmm_stage_params<R>(a, par_rows);
 
   LmmSobol sb;
   {
@@ -185,7 +240,8 @@ sdesim_lmm(const LmmArgs a) {
   for (int i = 0; i < NF; ++i) L[i] = (R)a.L0[i];
   R* ck = (R*)a.checkpoints;
   for (int n = 0; n < N; ++n) {
-    if (ADJ && (n % SEG) == 0 && live) {
+    if (MODE == 2) sde_tape_store<R, NF>(a.tape, n, S, voff, L);
+    if (MODE == 1 && (n % SEG) == 0 && live) {
       const long long slot = n / SEG;
 #pragma unroll
       for (int i = 0; i < NF; ++i) ck[(slot * NF + i) * S + sid] = L[i];
@@ -194,47 +250,14 @@ sdesim_lmm(const LmmArgs a) {
     lmm_noise<R>(rsrc, sb, n, eps);
     lmm_step<R>(LmmPar<R>{par_rows, n * a.zero}, L, eps, n / a.steps_per_period + 1, dt, sdt, tau);
   }
-
-  // Payoff and its derivative with respect to the terminal forwards.
-  R numer = R(1);
-#pragma unroll
-  for (int k = 0; k < NF; ++k)
-    if (k < e) numer = numer * (R(1) + tau * L[k]);
-  R P = R(1), A = R(0);
-  R Pj[NF];
-#pragma unroll
-  for (int j = 0; j < NF; ++j) {
-    Pj[j] = R(0);
-    if (j >= e) {
-      P = P / (R(1) + tau * L[j]);
-      Pj[j] = P;
-      A = A + tau * P;
-    }
+  if (MODE == 2) {
+    sde_tape_store<R, NF>(a.tape, N, S, voff, L);
+    return;
   }
-  const R U = K * A - (R(1) - P);
-  const R V = U > R(0) ? U / numer : R(0);
 
-  double acc[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; ++q) acc[q] = 0.0;
-  if (live) {
-    acc[0] = (double)V;
-    acc[1] = (double)V * (double)V;
-  }
-  if (ADJ) {
-    R Lb[NF];
-    // dV/dL_k: numeraire part for k < e; swap part for k >= e (only when in the money).
-    R tail = R(0);  // sum_{j >= k} P_j, built from the end
-#pragma unroll
-    for (int k = NF - 1; k >= 0; --k) {
-      if (k >= e) tail = tail + Pj[k];
-      const R op = R(1) + tau * L[k];
-      if (k < e) {
-        Lb[k] = -V * tau / op;
-      } else {
-        Lb[k] = U > R(0) ? (-K * tau * tau * tail - tau * P) / (op * numer) : R(0);
-      }
-    }
+  R Lb[NF];
+  const R V = lmm_payoff<R, MODE == 1>(L, e, tau, K, Lb);
+  if (MODE == 1) {
     // Lp holds the state after the step being reversed: L_N first, then each state read back.
     R Lp[NF];
 #pragma unroll
@@ -269,20 +292,73 @@ sdesim_lmm(const LmmArgs a) {
         for (int i = 0; i < NF; ++i) Lp[i] = Xc[i];
       }
     }
-    if (live) {
+  }
+  sde_reduce_store_chunked<NQ>(
+      [&](int q) {
+        return !live ? 0.0 : q == 0 ? (double)V : q == 1 ? (double)V * (double)V : MODE == 1 ? (double)Lb[q - 2] : 0.0;
+      },
+      a.partials + (long long)blockIdx.x * NQ);
+}
+
+// Reverse sweep of the tape form. L_N from the tape gives the payoff; then, for each step from
+// the last, the adjoint runs over the forwards from the last down, and as soon as forward i of
+// the state is used its register is refilled with forward i of the previous step's state (the
+// request after step 0 re-reads step 0 and is unused). The loads are issued in the order in
+// which the next step consumes them, so each has the rest of the step to arrive. The f32 kernel is
+// held to 3 waves per SIMD (168 VGPRs; it needs 169 unconstrained).
+template <typename R>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 1)))
+sdesim_lmm_reverse(const LmmArgs a) {
+  constexpr int NQ = 2 + NF;
+  const long long S = a.paths;
+  const int N = a.steps;
+  const long long s = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool live = s < S;
+  const sde_u32 voff = live ? (sde_u32)(s * (long long)sizeof(R)) : 0x80000000u;
+  const R dt = (R)a.dt, tau = (R)a.tau, K = (R)a.strike;
+
+  __shared__ LmmParRow<R> par_rows[NF];
+  lmm_stage_params<R>(a, par_rows);
+
+  R L[NF], Lb[NF], Lp[NF];
+  sde_tape_load<R, NF>(a.tape, N, S, voff, Lp);
+  const R V = lmm_payoff<R, true>(Lp, a.expiry_index, tau, K, Lb);
+  sde_tape_load<R, NF>(a.tape, N - 1, S, voff, L);
+  for (int n = N - 1; n >= 0; --n) {
+    const LmmPar<R> tab{par_rows, n * a.zero};
+    const int eta = n / a.steps_per_period + 1;
+    const __amdgpu_buffer_rsrc_t rs = sde_tape_rsrc<R, NF>(a.tape, n > 0 ? n - 1 : 0, S);
+    R G[F] = {R(0), R(0), R(0)};
 #pragma unroll
-      for (int i = 0; i < NF; ++i) acc[2 + i] = (double)Lb[i];
+    for (int i = NF - 1; i >= 0; --i) {
+      if (i >= eta) lmm_forward_adjoint<R>(tab.row(i), L[i], Lp[i], Lb[i], G, dt, tau);
+      Lp[i] = L[i];
+      L[i] = sde_tape_get<R>(rs, voff, i, S);
     }
   }
-  sde_block_reduce_store<NQ>(acc, a.partials + (long long)blockIdx.x * NQ);
+  sde_reduce_store_chunked<NQ>(
+      [&](int q) { return !live ? 0.0 : q == 0 ? (double)V : q == 1 ? (double)V * (double)V : (double)Lb[q - 2]; },
+      a.partials + (long long)blockIdx.x * NQ);
 }
 
 }  // namespace
 
-hipError_t sdesim_launch_lmm(int f32, int adjoint, int grid, const LmmArgs& a, hipStream_t stream) {
-  if (f32 && adjoint) hipLaunchKernelGGL((sdesim_lmm<float, true>), dim3(grid), dim3(256), 0, stream, a);
-  else if (f32) hipLaunchKernelGGL((sdesim_lmm<float, false>), dim3(grid), dim3(256), 0, stream, a);
-  else if (adjoint) hipLaunchKernelGGL((sdesim_lmm<double, true>), dim3(grid), dim3(256), 0, stream, a);
-  else hipLaunchKernelGGL((sdesim_lmm<double, false>), dim3(grid), dim3(256), 0, stream, a);
+// mode: 0 price, 1 adjoint with checkpoints and the segment tape, 2 adjoint with the state tape
+// (a.tape holds [steps + 1][40][paths]; a forward and a reverse kernel).
+hipError_t sdesim_launch_lmm(int f32, int mode, int grid, const LmmArgs& a, hipStream_t stream) {
+#define SDE_LMM(R, MODE)                                                                                  \
+  hipLaunchKernelGGL((sdesim_lmm<R, MODE>), dim3(grid), dim3(256), 0, stream, a);                            \
+  if (MODE == 2) hipLaunchKernelGGL((sdesim_lmm_reverse<R>), dim3(grid), dim3(256), 0, stream, a);          \
+  break;
+  switch ((f32 ? 10 : 0) + mode) {
+    case 12: SDE_LMM(float, 2)
+    case 11: SDE_LMM(float, 1)
+    case 10: SDE_LMM(float, 0)
+    case 2: SDE_LMM(double, 2)
+    case 1: SDE_LMM(double, 1)
+    case 0: SDE_LMM(double, 0)
+    default: return hipErrorInvalidValue;
+  }
+#undef SDE_LMM
   return hipGetLastError();
 }

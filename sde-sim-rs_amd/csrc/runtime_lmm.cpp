@@ -11,7 +11,7 @@
 #include "../../include/sdesim.h"
 #include "device/sdesim_args.hpp"
 
-hipError_t sdesim_launch_lmm(int f32, int adjoint, int grid, const LmmArgs& a, hipStream_t stream);
+hipError_t sdesim_launch_lmm(int f32, int mode, int grid, const LmmArgs& a, hipStream_t stream);
 hipError_t sdesim_launch_sum_partials(const double* partials, int nblocks, int nq, double* out, hipStream_t stream);
 int sdesim_set_error(int code, const std::string& msg);
 void sdesim_host_sobol_tables(int dims, int nslices, std::vector<unsigned int>& out);
@@ -46,8 +46,9 @@ struct sdesim_lmm {
   unsigned int* tab = nullptr;    // slice 0, [steps][64][3], mask complements folded in
   unsigned int* tabup = nullptr;  // slices 1-2, [2][256][steps * 3]
   int tab_words = 0;
-  void* ckpt = nullptr;
-  void* tape = nullptr;
+  int storage = SDESIM_STORAGE_CHECKPOINTS;  // resolved at creation: TAPE or CHECKPOINTS
+  void* ckpt = nullptr;                      // checkpoints (CHECKPOINTS)
+  void* tape = nullptr;                      // the segment tape (CHECKPOINTS) or the state tape (TAPE)
   double* pard = nullptr;
   float* parf = nullptr;
   double* partials = nullptr;
@@ -71,6 +72,8 @@ int sdesim_lmm_create(const sdesim_lmm_spec* spec, sdesim_lmm** out) {
       throw LFail{SDESIM_EVALUE, "steps must equal steps_per_period * expiry_index"};
     if (s.paths < 1 || s.paths > (1LL << 22)) throw LFail{SDESIM_EVALUE, "paths must be in [1, 2^22]"};
     if (!(s.tenor > 0.0)) throw LFail{SDESIM_EVALUE, "tenor must be > 0"};
+    if (s.storage < SDESIM_STORAGE_AUTO || s.storage > SDESIM_STORAGE_CHECKPOINTS)
+      throw LFail{SDESIM_EVALUE, "storage must be SDESIM_STORAGE_AUTO, _TAPE or _CHECKPOINTS"};
     std::unique_ptr<sdesim_lmm> m(new sdesim_lmm());
     m->spec = s;
     m->grid = (s.paths + 255) / 256;
@@ -102,8 +105,23 @@ int sdesim_lmm_create(const sdesim_lmm_spec* spec, sdesim_lmm** out) {
     m->tab_words = (int)t0.size();
     const size_t rb = s.precision == SDESIM_F32 ? 4 : 8;
     const size_t slots = (size_t)(s.steps + kSeg - 1) / kSeg;
-    lk(hipMalloc(&m->ckpt, slots * kNF * (size_t)s.paths * rb), "hipMalloc(checkpoints)");
-    lk(hipMalloc(&m->tape, (size_t)kSeg * kNF * (size_t)s.paths * rb), "hipMalloc(segment tape)");
+    // One step of the state tape is 40 * paths * rb <= 1.34 GB (paths <= 2^22): one buffer
+    // resource always covers it.
+    const size_t tape = (size_t)kNF * (size_t)s.paths * rb * ((size_t)s.steps + 1);  // L_0 .. L_N
+    size_t free_b = 0, total_b = 0;
+    lk(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+    if (s.storage == SDESIM_STORAGE_TAPE) {
+      if (tape + (64u << 20) > free_b) throw LFail{SDESIM_ERUNTIME, "the tape does not fit in free device memory"};
+      m->storage = SDESIM_STORAGE_TAPE;
+    } else if (s.storage == SDESIM_STORAGE_AUTO && tape + (64u << 20) <= free_b / 2) {
+      m->storage = SDESIM_STORAGE_TAPE;
+    }
+    if (m->storage == SDESIM_STORAGE_TAPE) {
+      lk(hipMalloc(&m->tape, tape), "hipMalloc(state tape)");
+    } else {
+      lk(hipMalloc(&m->ckpt, slots * kNF * (size_t)s.paths * rb), "hipMalloc(checkpoints)");
+      lk(hipMalloc(&m->tape, (size_t)kSeg * kNF * (size_t)s.paths * rb), "hipMalloc(segment tape)");
+    }
     lk(hipMalloc(&m->pard, sizeof(double) * kNF * (1 + kF)), "hipMalloc(parameters)");
     lk(hipMalloc(&m->parf, sizeof(float) * kNF * (1 + kF)), "hipMalloc(parameters)");
     lk(hipMalloc(&m->partials, sizeof(double) * kNQ * m->grid), "hipMalloc(partials)");
@@ -113,6 +131,8 @@ int sdesim_lmm_create(const sdesim_lmm_spec* spec, sdesim_lmm** out) {
 }
 
 void sdesim_lmm_free(sdesim_lmm* m) { delete m; }
+
+int32_t sdesim_lmm_storage(const sdesim_lmm* m) { return m ? m->storage : -1; }
 
 int sdesim_lmm_eval(sdesim_lmm* m, const double* L0, const double* vol, const double* loadings, int32_t adjoint,
                     double* out, void* stream) {
@@ -152,7 +172,8 @@ int sdesim_lmm_eval(sdesim_lmm* m, const double* L0, const double* vol, const do
     a.steps = s.steps;
     a.steps_per_period = s.steps_per_period;
     a.expiry_index = s.expiry_index;
-    lk(sdesim_launch_lmm(s.precision == SDESIM_F32 ? 1 : 0, adjoint ? 1 : 0, (int)m->grid, a, st), "lmm launch");
+    const int mode = !adjoint ? 0 : m->storage == SDESIM_STORAGE_TAPE ? 2 : 1;
+    lk(sdesim_launch_lmm(s.precision == SDESIM_F32 ? 1 : 0, mode, (int)m->grid, a, st), "lmm launch");
     lk(sdesim_launch_sum_partials(m->partials, (int)m->grid, kNQ, m->sums, st), "lmm reduce");
     std::vector<double> sums(kNQ);
     lk(hipMemcpyAsync(sums.data(), m->sums, sizeof(double) * kNQ, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");

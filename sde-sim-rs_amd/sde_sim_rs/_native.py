@@ -75,6 +75,8 @@ class LmmSpec(ctypes.Structure):
         ("tenor", ctypes.c_double),
         ("strike", ctypes.c_double),
         ("scramble_seed", ctypes.c_uint64),
+        ("storage", ctypes.c_int32),
+        ("pad_", ctypes.c_int32),
     ]
 
 
@@ -133,6 +135,7 @@ SIGNATURES = {
                            [ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
     "sdesim_lmm_create": (ctypes.c_int, [ctypes.POINTER(LmmSpec), ctypes.POINTER(ctypes.c_void_p)]),
     "sdesim_lmm_free": (None, [ctypes.c_void_p]),
+    "sdesim_lmm_storage": (ctypes.c_int32, [ctypes.c_void_p]),
     "sdesim_lmm_eval": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_double)] * 3 +
                         [ctypes.c_int32, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
     "sdesim_last_error": (ctypes.c_char_p, []),

@@ -33,12 +33,20 @@ def forward_rate_config(seed=5, tenor=0.25):
 
 class ForwardRateModel:
     """Receiver swaption on the swap [T_e, T_40] (e = expiry_index), fixed rate `strike`,
-    priced under the spot measure with `steps` log-Euler steps to T_e."""
+    priced under the spot measure with `steps` log-Euler steps to T_e.
+
+    `storage` picks how the adjoint keeps the forward states: "tape" (all steps + 1 states of the
+    40 forwards in device memory), "checkpoints" (every 16th state, each segment recomputed once
+    during the reverse sweep) or "auto" (the tape when it fits in half of the free device
+    memory). `self.storage` is the form in use."""
 
     def __init__(self, steps=160, paths=1 << 20, expiry_index=4, strike=0.03, tenor=0.25, scramble_seed=5,
-                 precision="f32"):
+                 precision="f32", storage="auto"):
         if steps % expiry_index:
             raise ValueError("steps must be a multiple of expiry_index")
+        storages = {"auto": nat.STORAGE_AUTO, "tape": nat.STORAGE_TAPE, "checkpoints": nat.STORAGE_CHECKPOINTS}
+        if storage not in storages:
+            raise ValueError("storage must be 'auto', 'tape' or 'checkpoints'")
         spec = nat.LmmSpec()
         spec.forwards, spec.factors = NF, NFACT
         spec.steps = steps
@@ -49,11 +57,13 @@ class ForwardRateModel:
         spec.tenor = tenor
         spec.strike = strike
         spec.scramble_seed = int(scramble_seed) & (2 ** 64 - 1)
+        spec.storage = storages[storage]
         self.steps, self.paths, self.expiry_index, self.strike, self.tenor = steps, paths, expiry_index, strike, tenor
         self.scramble_seed = scramble_seed
         h = ctypes.c_void_p()
         nat.check(nat.lib.sdesim_lmm_create(ctypes.byref(spec), ctypes.byref(h)))
         self._h = h
+        self.storage = "tape" if nat.lib.sdesim_lmm_storage(h) == nat.STORAGE_TAPE else "checkpoints"
 
     def evaluate(self, L0, vol, loadings, adjoint=True, stream=None):
         L0 = np.ascontiguousarray(L0, dtype=np.float64)
