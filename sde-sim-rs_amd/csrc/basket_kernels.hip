@@ -38,14 +38,18 @@ struct BasketConst {
   static constexpr int NQ = 2 + 3 * D + NL;    // reduced quantities: payoff, payoff^2, gradients
 };
 
-// The D f64 draws of step n: draws n*D .. n*D + D - 1 of the path's stream.
+// The words (low, high) of the D f64 draws of step n: draws n*D .. n*D + D - 1 of the path's
+// stream (draw j is words 2j and 2j + 1).
 template <int D>
-__device__ __forceinline__ void basket_draws(const sde_u32 (&key)[8], int n, double (&u)[D]) {
+__device__ __forceinline__ void basket_draws(const sde_u32 (&key)[8], int n, sde_u32 (&lo)[D], sde_u32 (&hi)[D]) {
   if constexpr (D == 8) {
     sde_u32 blk[16];
     sde_chacha_block<8>(key, (sde_u64)n, blk);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) u[k] = sde_u01_from_words(blk[2 * k], blk[2 * k + 1]);
+    for (int k = 0; k < 8; ++k) {
+      lo[k] = blk[2 * k];
+      hi[k] = blk[2 * k + 1];
+    }
   } else {
 #pragma unroll
     for (int k = 0; k < D; ++k) {
@@ -53,13 +57,14 @@ __device__ __forceinline__ void basket_draws(const sde_u32 (&key)[8], int n, dou
       sde_u32 blk[16];
       sde_chacha_block<8>(key, (sde_u64)(j >> 3), blk);
       const int w = (int)(j & 7);
-      sde_u32 lo = blk[0], hi = blk[1];
+      sde_u32 l = blk[0], h = blk[1];
 #pragma unroll
       for (int q = 1; q < 8; ++q) {
-        lo = (w == q) ? blk[2 * q] : lo;
-        hi = (w == q) ? blk[2 * q + 1] : hi;
+        l = (w == q) ? blk[2 * q] : l;
+        h = (w == q) ? blk[2 * q + 1] : h;
       }
-      u[k] = sde_u01_from_words(lo, hi);
+      lo[k] = l;
+      hi[k] = h;
     }
   }
 }
@@ -72,10 +77,10 @@ struct BasketStep {
 template <typename R, int D>
 __device__ __forceinline__ void basket_noise(const sde_u32 (&key)[8], int n, const R (&L)[D][D],
                                              BasketStep<R, D>& st) {
-  double u[D];
-  basket_draws<D>(key, n, u);
+  sde_u32 lo[D], hi[D];
+  basket_draws<D>(key, n, lo, hi);
 #pragma unroll
-  for (int k = 0; k < D; ++k) st.eps[k] = sde_psi<R>(u[k]);
+  for (int k = 0; k < D; ++k) st.eps[k] = sde_psi_words<R>(lo[k], hi[k]);
 #pragma unroll
   for (int i = 0; i < D; ++i) {
     R z = R(0);

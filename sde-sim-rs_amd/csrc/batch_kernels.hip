@@ -90,8 +90,7 @@ __global__ void __launch_bounds__(256) sdesim_batch(const SdeInstance* __restric
         if (n0 + i < N) {
 #pragma unroll
           for (int j = 0; j < PPL; ++j) {
-            const double u = sde_u01_from_words(W[j][2 * i], W[j][2 * i + 1]);
-            const R z = sde_psi<R>(u);
+            const R z = sde_psi_words<R>(W[j][2 * i], W[j][2 * i + 1]);
             x[j] = sde_euler_builtin<R>(kind, x[j], p0, p1, p2, dt, sdt, z);
           }
         }

@@ -129,6 +129,39 @@ __device__ __forceinline__ float sde_inv_normal_f32(double p) {
   return (p < 0.5) ? -x : x;
 }
 
+// The same f32 inverse normal of the rand f64 uniform u = (w >> 11) 2^-53 of the 64-bit word
+// w = hi:lo (sde_u01_from_words), formed from the words without f64 arithmetic on the fast path.
+// With m the sign extension of bit 31 of hi (set exactly when u >= 1/2),
+//   q' = (hi ^ m) 2^-32 + (lo ^ m) 2^-64
+// is q = min(u, 1 - u) up to 2^-53 (the 11 low bits that the f64 conversion drops and, for
+// u >= 1/2, the complement's offset of one unit of 2^-53): below 2^-31 relative for
+// q >= 2.87e-7, inside the rounding of the two f32 conversions. q' < 2.87e-7 (|z| > 5) takes the
+// f64 path with the exact q.
+__device__ __forceinline__ float sde_inv_normal_f32_words(sde_u32 lo, sde_u32 hi) {
+  const sde_u32 m = (sde_u32)((int)hi >> 31);
+  const float q = __builtin_fmaf((float)(hi ^ m), 0x1.0p-32f, (float)(lo ^ m) * 0x1.0p-64f);
+  float x;
+  if (q < 2.87e-7f) {
+    const double p = sde_u01_from_words(lo, hi);
+    x = sde_inv_normal_tail(p < 0.5 ? p : 1.0 - p);
+  } else {
+    const float t = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(q));
+    const float num = (0.010328f * t + 0.802853f) * t + 2.515517f;
+    const float den = ((0.001308f * t + 0.189269f) * t + 1.432788f) * t + 1.0f;
+    x = t - num * __builtin_amdgcn_rcpf(den);
+  }
+  return m ? x : -x;
+}
+
+// Psi of the pseudo-random draw held as its two ChaCha words (lo, hi).
+template <typename R> __device__ __forceinline__ R sde_psi_words(sde_u32 lo, sde_u32 hi);
+template <> __device__ __forceinline__ double sde_psi_words<double>(sde_u32 lo, sde_u32 hi) {
+  return sde_inv_normal(sde_u01_from_words(lo, hi));
+}
+template <> __device__ __forceinline__ float sde_psi_words<float>(sde_u32 lo, sde_u32 hi) {
+  return sde_inv_normal_f32_words(lo, hi);
+}
+
 // A uniform from the digital-XOR-scrambled Sobol stream, kept as the complement yc = ~y of its
 // 32-bit integer y; its value is (y + 0.5) * 2^-32, strictly inside (0, 1). Bit 31 of yc is set
 // exactly when u < 1/2, which is the sign bit of Psi(u) and of sign(u - 1/2) sqrt(dt).
