@@ -319,3 +319,34 @@ def test_fp32_q_from_signed_word():
     small = qi < (1 << 23)
     np.testing.assert_array_equal(q[small].astype(np.float64), exact[small])
     assert np.all(np.abs(q.astype(np.float64) - exact) <= 2.0**-23 * exact)
+
+
+def test_fp32_q_from_chacha_words():
+    """The f32 device forms q = min(u, 1 - u) of rand's f64 uniform u = (w >> 11) 2^-53 of the
+    64-bit word w = hi:lo straight from the words: with m the sign extension of bit 31 of hi,
+    q' = (f32)(hi ^ m) 2^-32 + (f32)(lo ^ m) 2^-64. Checked against the exact q: within 2^-22
+    relative wherever q' takes the f32 path (q' >= 2.87e-7), and m set exactly when u >= 1/2 (the
+    sign of Psi)."""
+    rng = np.random.default_rng(9)
+    n = 400000
+    hi = rng.integers(0, 1 << 32, n, dtype=np.uint64)
+    lo = rng.integers(0, 1 << 32, n, dtype=np.uint64)
+    # Near 0, 1/2 and 1, and around the tail threshold, on both halves.
+    t = int(2.87e-7 * 2.0**32)
+    hi[:4096] = np.arange(4096) % 64
+    hi[4096:8192] = (1 << 32) - 1 - np.arange(4096) % 64
+    hi[8192:12288] = (1 << 31) + np.arange(-2048, 2048)
+    hi[12288:14336] = t + np.arange(-1024, 1024)
+    hi[14336:16384] = (1 << 32) - 1 - t + np.arange(-1024, 1024)
+    w = (hi << np.uint64(32)) | lo
+    u = (w >> np.uint64(11)).astype(np.float64) * 2.0**-53
+    q = np.minimum(u, 1.0 - u)
+    m = np.where(hi >= (1 << 31), np.uint64(0xFFFFFFFF), np.uint64(0))
+    fh = (hi ^ m).astype(np.uint32).astype(np.float32).astype(np.float64)
+    fl = ((lo ^ m).astype(np.uint32).astype(np.float32).astype(np.float64)) * 2.0**-64
+    qp = (fh * 2.0**-32 + fl).astype(np.float32).astype(np.float64)
+    np.testing.assert_array_equal(m != 0, u >= 0.5)
+    fast = qp >= np.float32(2.87e-7)
+    assert np.all(np.abs(qp[fast] - q[fast]) <= 2.0**-22 * q[fast])
+    # Every draw the f32 path would mis-place around the threshold lies within that tolerance of it.
+    assert np.all(q[~fast] < 2.87e-7 * (1 + 2.0**-22))
