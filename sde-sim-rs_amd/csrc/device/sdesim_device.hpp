@@ -162,6 +162,16 @@ template <> __device__ __forceinline__ float sde_psi_words<float>(sde_u32 lo, sd
   return sde_inv_normal_f32_words(lo, hi);
 }
 
+// A pseudo-random uniform kept as its two ChaCha words: u = (hi:lo >> 11) 2^-53. u > 1/2 exactly
+// when hi:lo >= 2^63 + 2^11, i.e. hi > 2^31, or hi = 2^31 and lo >= 2^11.
+struct SdeUnifW {
+  sde_u32 lo, hi;
+};
+__device__ __forceinline__ double sde_uval(SdeUnifW u) { return sde_u01_from_words(u.lo, u.hi); }
+__device__ __forceinline__ bool sde_gt_half(SdeUnifW u) {
+  return u.hi > 0x80000000u || (u.hi == 0x80000000u && u.lo >= 2048u);
+}
+
 // A uniform from the digital-XOR-scrambled Sobol stream, kept as the complement yc = ~y of its
 // 32-bit integer y; its value is (y + 0.5) * 2^-32, strictly inside (0, 1). Bit 31 of yc is set
 // exactly when u < 1/2, which is the sign bit of Psi(u) and of sign(u - 1/2) sqrt(dt).
@@ -266,12 +276,15 @@ template <> __device__ __forceinline__ float sde_psi<float>(SdeUnifX u) { return
 __device__ __forceinline__ double sde_dw(double u, double sdt) { return sdt * sde_inv_normal(u); }
 __device__ __forceinline__ float sde_dw(double u, float sdt) { return sdt * sde_inv_normal_f32(u); }
 __device__ __forceinline__ double sde_dw(SdeUnifX u, double sdt) { return sdt * sde_inv_normal(sde_uval(u)); }
+__device__ __forceinline__ double sde_dw(SdeUnifW u, double sdt) { return sdt * sde_inv_normal(sde_uval(u)); }
+__device__ __forceinline__ float sde_dw(SdeUnifW u, float sdt) { return sdt * sde_inv_normal_f32_words(u.lo, u.hi); }
 __device__ __forceinline__ float sde_dw(SdeUnifX u, float sdt) {
   return sde_sign_select(u.yc, sdt * (float)SDE_PSI_C, u.sm) * sde_inv_normal_f32_x_scaled(u.yc, u.tail);
 }
 
 template <typename R> __device__ __forceinline__ R sde_signed(double u, R sdt) { return (u > 0.5 ? R(1) : R(-1)) * sdt; }
 __device__ __forceinline__ double sde_signed(SdeUnifX u, double sdt) { return (sde_gt_half(u) ? 1.0 : -1.0) * sdt; }
+template <typename R> __device__ __forceinline__ R sde_signed(SdeUnifW u, R sdt) { return (sde_gt_half(u) ? R(1) : R(-1)) * sdt; }
 __device__ __forceinline__ float sde_signed(SdeUnifX u, float sdt) { return sde_sign_select(u.yc, sdt, u.sm2); }
 
 // Sequential inverse Poisson CDF capped at k = 200 (f64, as the reference).
@@ -394,6 +407,7 @@ typedef sde_u32 SdeU32x2 __attribute__((ext_vector_type(2)));
 
 template <int RNG> struct SdeUnifOf { typedef double type; };
 template <> struct SdeUnifOf<SDE_RNG_SOBOL_XOR> { typedef SdeUnifX type; };
+template <> struct SdeUnifOf<SDE_RNG_PSEUDO> { typedef SdeUnifW type; };
 
 template <typename R>
 struct SdeGrid {
@@ -642,10 +656,7 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
 #pragma unroll
           for (int k = 0; k < MM; ++k) {
             double v = 0.5;
-            if (RNG == SDE_RNG_PSEUDO && M > 0) {
-              const int w = 2 * (i * M + k);
-              v = sde_u01_from_words(W[j][USE_CHACHA ? w : 0], W[j][USE_CHACHA ? w + 1 : 0]);
-            } else if (RNG == SDE_RNG_SOBOL_SHIFT && M > 0) {
+            if (RNG == SDE_RNG_SOBOL_SHIFT && M > 0) {
               const int w = 2 * (i * M + k);
               const double sh = sde_u01_from_words(W[j][USE_CHACHA ? w : 0], W[j][USE_CHACHA ? w + 1 : 0]);
               const double y = (double)nx[j][k] * 0x1.0p-32 + sh;
@@ -658,6 +669,10 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
               u[j][k].tail = a.tail_tab;
               u[j][k].sm = smask;
               u[j][k].sm2 = smask2;
+            } else if constexpr (RNG == SDE_RNG_PSEUDO) {
+              const int w = 2 * (i * M + k);
+              u[j][k].lo = W[j][USE_CHACHA ? w : 0];
+              u[j][k].hi = W[j][USE_CHACHA ? w + 1 : 0];
             } else {
               u[j][k] = v;
             }

@@ -162,6 +162,19 @@ def test_fp32_xor_tail_increments(oracle):
     np.testing.assert_allclose(out["f32"], out["f64"], rtol=0, atol=2e-4)
 
 
+@pytest.mark.parametrize("scheme", ["euler", "runge-kutta"])
+def test_fp32_pseudo_matches_oracle(oracle, scheme):
+    """FP32 pseudo-random paths (the inverse normal formed from each draw's two ChaCha words, the
+    RK sign from the words) follow the oracle's FP64 paths to FP32 accuracy: GBM, 252 steps,
+    5000 paths (a partial last block), every value within 5e-5 relative."""
+    times = np.linspace(0.0, 1.0, 253)
+    S = 5000
+    print("case: fp32 pseudo %s gbm S=%d N=252" % (scheme, S), flush=True)
+    _, f = _filtration(GBM, times, S, {"X": 100.0}, "pseudo", scheme, seed=42, precision="f32")
+    ref = oracle.simulate(GBM, times, S, {"X": 100.0}, scheme=scheme, rng="pseudo", seed=42)
+    _close(f.transpose(2, 0, 1), ref, rtol=5e-5, atol=0.0)
+
+
 def test_terminal_moments_match_filtration():
     import sde_sim_rs
     times = np.linspace(0.0, 1.0, 101)
