@@ -226,3 +226,38 @@ def test_dataframe_layout_matches_oracle(oracle):
     assert tab.column("scenario").to_pylist()[:13] == [0] * 12 + [1]
     assert tab.column("process_name").to_pylist()[:4] == ["X0", "X1", "X2", "X0"]
     assert tab.column("time").to_pylist()[:4] == [0.0, 0.0, 0.0, 0.25]
+
+
+def test_single_time_point_and_single_scenario(oracle):
+    """Edge shapes: one time point (no steps: the filtration is the initial row, the terminal
+    moments are those of a constant) and one scenario (a single live lane in the grid)."""
+    import sde_sim_rs
+    print("case: one time point; one scenario", flush=True)
+    f = sde_sim_rs.simulate(HESTON, [0.0], 3, {"S": 100.0, "v": 0.04}, "sobol", "runge-kutta", seed=1,
+                            output="filtration")
+    torch.cuda.synchronize()
+    got = f.cpu().numpy()
+    assert got.shape == (1, 2, 3)
+    np.testing.assert_array_equal(got[0, 0], 100.0)
+    np.testing.assert_array_equal(got[0, 1], 0.04)
+    m = sde_sim_rs.simulate(GBM, [0.0], 7, {"X": 100.0}, "pseudo", "euler", seed=1, output="terminal")
+    assert m["X"][0] == 100.0 and m["X"][1] == 0.0
+    times = np.linspace(0.0, 1.0, 65)
+    _, f1 = _filtration(HESTON, times, 1, {"S": 100.0, "v": 0.04}, "pseudo", "runge-kutta", seed=5)
+    ref = oracle.simulate(HESTON, times, 1, {"S": 100.0, "v": 0.04}, scheme="runge-kutta", rng="pseudo", seed=5)
+    _close(f1.transpose(2, 0, 1), ref)
+
+
+@pytest.mark.parametrize("precision,rtol", [("f64", 1e-12), ("f32", 2e-5)])
+def test_nonuniform_grid_xor(oracle, precision, rtol):
+    """A non-uniform time grid (per-step dt and sqrt(dt) arrays instead of the uniform-grid
+    constants), Sobol-XOR RK: FP64 at the parity bar, FP32 to FP32 accuracy (GBM, 3000 paths)."""
+    rng = np.random.default_rng(4)
+    times = np.concatenate([[0.0], np.sort(rng.uniform(0.0, 1.0, 39))])
+    S = 3000
+    print("case: nonuniform grid xor rk %s S=%d T=%d" % (precision, S, len(times)), flush=True)
+    _, f = _filtration(GBM, times, S, {"X": 100.0}, "sobol", "runge-kutta", scramble="xor", scramble_seed=9,
+                       precision=precision)
+    ref = oracle.simulate(GBM, times, S, {"X": 100.0}, scheme="runge-kutta", rng="sobol", scramble="xor",
+                          scramble_seed=9)
+    _close(f.transpose(2, 0, 1), ref, rtol=rtol, atol=0.0 if precision == "f32" else 1e-12)
