@@ -127,7 +127,7 @@ def main():
     ap.add_argument("--cpu-paths", type=int, default=1 << 18)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true", help="run a 1 GiB device copy first (HBM counter calibration)")
-    ap.add_argument("--only", default="", help="comma list of legs to run: f32,f64,term,batch,adjoint,lmm")
+    ap.add_argument("--only", default="", help="comma list of legs to run: f32,long,f64,term,batch,adjoint,lmm")
     ap.add_argument("--adjoint-storage", default="auto", choices=("auto", "tape", "checkpoints"),
                     help="state storage of the basket and forward-rate adjoints")
     args = ap.parse_args()
@@ -143,7 +143,7 @@ def main():
 
     S, N = args.paths, args.nsteps
     times = np.linspace(0.0, 1.0, N + 1)
-    legs = set(args.only.split(",")) if args.only else {"f32", "f64", "term", "batch", "adjoint", "lmm"}
+    legs = set(args.only.split(",")) if args.only else {"f32", "long", "f64", "term", "batch", "adjoint", "lmm"}
     common = dict(seed=0, scramble="xor", scramble_seed=7, paths_per_lane=args.ppl, block_size=args.block)
     uni = sde_sim_rs.Universe(HESTON)
     res = {}
@@ -159,6 +159,24 @@ def main():
         f32 = p32.new_filtration()
         res["f32"] = _time_plan(p32, f32, args.steps, args.warmup)
         res["fill_ms"] = _time_fill(f32, args.steps, args.warmup)
+        if "long" in legs:
+            # The long layout's value column [S][T][P] f64 from the same filtration (one
+            # LDS-tiled transpose launch): reads 4 B and writes 8 B per value.
+            stream = torch.cuda.current_stream()
+            out = p32.to_long(f32, stream=stream)
+            reps = max(1, args.steps // 4)
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            a.record(stream)
+            for _ in range(reps):
+                out = p32.to_long(f32, stream=stream)
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b) / reps
+            nbytes = f32.numel() * (4 + 8)
+            res["long"] = {"workload": "filtration [1025][2][2^20] f32 -> long value column [2^20][1025][2] f64",
+                           "ms": ms, "bytes": nbytes, "gbs": nbytes / (ms * 1e-3) / 1e9}
+            del out
         del f32
     if "f64" in legs:
         p64 = sde_sim_rs.Plan(uni, times, S, INIT, "sobol", "runge-kutta", precision="f64", **common)
@@ -294,6 +312,8 @@ def main():
                                      "valu_per_path_step": vi, "trans_per_path_step": tr,
                                      "kernel": "sdesim_rk_f32_sobolxor_term"}
         line["terminal_only"] = term
+    if "long" in res:
+        line["long_layout"] = res["long"]
     if batch:
         line["batched"] = batch
     if adj:

@@ -42,24 +42,67 @@ __global__ void __launch_bounds__(256) sdesim_finalize(const double* __restrict_
   }
 }
 
-// Tile of 64 rows (time-process pairs) x 64 scenarios; 256 threads = 64 x 4.
+// Long layout: [rows][S] (rows = T * P) to [S][rows] f64. A block moves a tile of 256 rows x W
+// scenarios through LDS (W = 32 for f32 input, 16 for f64: 128 bytes of one row per 8 lanes):
+//  * reads: each lane loads 16 bytes (4 f32 or 2 f64 consecutive scenarios), a whole 128-byte
+//    line of a row per 8 lanes;
+//  * writes: for each scenario, the tile's 256 rows as f64 (2 KB contiguous), each lane storing
+//    two adjacent rows in one 16-byte non-temporal store (the output is written once and not
+//    read back by this kernel);
+//  * the LDS row pitch is W + 1 elements.
+// Among the layouts measured on 2050 x 2^20 values (64 x 128 up to 512 x 16 tiles, 8- or 16-byte
+// stores, default or non-temporal policy) this one moved the most bytes per second. Tiles that are
+// not whole (the last rows or scenarios), or data whose rows are not 16-byte aligned (S not a
+// multiple of the vector width, an odd row count), take element-wise guarded accesses.
 template <typename T>
 __global__ void __launch_bounds__(256) sdesim_to_long(const T* __restrict__ in, long long S, int rows,
                                                       double* __restrict__ out) {
-  __shared__ double tile[64][65];
-  const long long s0 = (long long)blockIdx.x * 64;
-  const int r0 = blockIdx.y * 64;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-  for (int r = ty; r < 64; r += 4) {
+  constexpr int VEC = 16 / (int)sizeof(T);
+  constexpr int W = 8 * VEC;
+  constexpr int RT = 256;
+  __shared__ T tile[RT][W + 1];
+  const long long s0 = (long long)blockIdx.x * W;
+  const int r0 = blockIdx.y * RT;
+  const int t = threadIdx.x;
+  const bool whole = r0 + RT <= rows && s0 + W <= S && S % VEC == 0;
+#pragma unroll
+  for (int pass = 0; pass < RT / 32; ++pass) {
+    const int r = pass * 32 + (t >> 3);  // 32 rows per pass, 8 lanes per row
+    const int c = (t & 7) * VEC;
     const int row = r0 + r;
-    const long long s = s0 + tx;
-    if (row < rows && s < S) tile[r][tx] = (double)in[(long long)row * S + s];
+    const long long s = s0 + c;
+    T v[VEC];
+    if (whole) {
+      typedef T TV __attribute__((ext_vector_type(VEC)));
+      const TV x = *(const TV*)(in + (long long)row * S + s);
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) v[e] = x[e];
+    } else {
+#pragma unroll
+      for (int e = 0; e < VEC; ++e) v[e] = (row < rows && s + e < S) ? in[(long long)row * S + s + e] : T(0);
+    }
+#pragma unroll
+    for (int e = 0; e < VEC; ++e) tile[r][c + e] = v[e];
   }
   __syncthreads();
-  for (int c = ty; c < 64; c += 4) {
+  typedef double D2 __attribute__((ext_vector_type(2)));
+  const int wave = t >> 6, lane = t & 63;
+  const bool pairs = whole && rows % 2 == 0;
+  for (int c = wave; c < W; c += 4) {
     const long long s = s0 + c;
-    const int row = r0 + tx;
-    if (row < rows && s < S) out[s * rows + row] = tile[tx][c];
+#pragma unroll
+    for (int k = 0; k < RT / 128; ++k) {
+      const int r = k * 128 + 2 * lane;
+      const int row = r0 + r;
+      if (pairs) {
+        const D2 v = {(double)tile[r][c], (double)tile[r + 1][c]};
+        __builtin_nontemporal_store(v, (D2*)(out + s * rows + row));
+      } else {
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+          if (row + e < rows && s < S) __builtin_nontemporal_store((double)tile[r + e][c], out + s * rows + row + e);
+      }
+    }
   }
 }
 
@@ -73,7 +116,8 @@ hipError_t sdesim_launch_finalize(const double* partials, int nblocks, int kq, l
 
 hipError_t sdesim_launch_to_long(const void* filtration, int f32, long long S, int rows, double* out,
                                  hipStream_t stream) {
-  dim3 grid((unsigned)((S + 63) / 64), (unsigned)((rows + 63) / 64));
+  const long long W = f32 ? 32 : 16;  // scenarios per tile (256 rows per tile)
+  dim3 grid((unsigned)((S + W - 1) / W), (unsigned)((rows + 255) / 256));
   if (grid.y > 65535u) return hipErrorInvalidValue;
   if (f32)
     hipLaunchKernelGGL(sdesim_to_long<float>, grid, dim3(256), 0, stream, (const float*)filtration, S, rows, out);

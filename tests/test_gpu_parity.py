@@ -261,3 +261,25 @@ def test_nonuniform_grid_xor(oracle, precision, rtol):
     ref = oracle.simulate(GBM, times, S, {"X": 100.0}, scheme="runge-kutta", rng="sobol", scramble="xor",
                           scramble_seed=9)
     _close(f.transpose(2, 0, 1), ref, rtol=rtol, atol=0.0 if precision == "f32" else 1e-12)
+
+
+@pytest.mark.parametrize("model,T,precision,S", [("heston", 300, "f32", 300), ("heston", 300, "f64", 300),
+                                                 ("heston", 300, "f32", 301), ("gbm", 301, "f32", 4096),
+                                                 ("heston", 70, "f32", 4096)])
+def test_long_layout_transpose(model, T, precision, S):
+    """The long layout's value column ([S][T][P] f64) is the exact transpose of the device
+    filtration ([T][P][S]), widened to f64: whole 256-row tiles and partial tiles in both
+    directions, a path count that is not a multiple of the vector width, and an odd row count
+    (element-wise accesses)."""
+    eqs, init = (HESTON, {"S": 100.0, "v": 0.04}) if model == "heston" else (GBM, {"X": 100.0})
+    times = np.linspace(0.0, 1.0, T)
+    print("case: long layout %s %s S=%d T=%d" % (model, precision, S, T), flush=True)
+    plan, _ = _filtration(eqs, times, S, init, "pseudo", "euler", seed=3, precision=precision)
+    dev = plan.new_filtration()
+    plan.run(dev)
+    out = plan.to_long(dev)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    want = dev.cpu().numpy().astype(np.float64).transpose(2, 0, 1)
+    assert got.dtype == np.float64 and got.shape == want.shape
+    np.testing.assert_array_equal(got, want)
