@@ -350,3 +350,24 @@ def test_fp32_q_from_chacha_words():
     assert np.all(np.abs(qp[fast] - q[fast]) <= 2.0**-22 * q[fast])
     # Every draw the f32 path would mis-place around the threshold lies within that tolerance of it.
     assert np.all(q[~fast] < 2.87e-7 * (1 + 2.0**-22))
+
+
+@pytest.mark.parametrize("chunk_rows", [1 << 24, 7, 12, 1])
+def test_long_table_layout_and_chunks(chunk_rows):
+    """The long table's columns, dtypes and row order (scenario -> time -> process), the same
+    whatever the chunking of the columns (whole scenarios per chunk)."""
+    from sde_sim_rs.api import long_table
+    names, times, S = ["X0", "X1", "X2"], [0.0, 0.25, 1.0], 4
+    values = np.arange(S * len(times) * len(names), dtype=np.float64).reshape(S, len(times), len(names)) / 7.0
+    tab = long_table(names, times, S, values, chunk_rows=chunk_rows)
+    tab = tab if hasattr(tab, "column") else tab.to_arrow()
+    assert tab.column_names == ["scenario", "time", "process_name", "value"]
+    assert [str(f.type) for f in tab.schema] == ["int32", "double", "string", "double"]
+    assert tab.num_rows == S * len(times) * len(names)
+    sc, tm, pn, vv = (tab.column(c).to_pylist() for c in tab.column_names)
+    k = 0
+    for s in range(S):
+        for ti, t in enumerate(times):
+            for pi, n in enumerate(names):
+                assert (sc[k], tm[k], pn[k], vv[k]) == (s, t, n, values[s, ti, pi])
+                k += 1
