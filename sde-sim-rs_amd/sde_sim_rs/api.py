@@ -78,9 +78,12 @@ class Universe:
         return x0
 
     def __del__(self):
+        # At interpreter exit the module globals may already be cleared (the process releases
+        # the device memory itself then).
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            nat.lib.sdesim_free_universe(h)
+        lib = getattr(nat, "lib", None)
+        if h is not None and h.value and lib is not None:
+            lib.sdesim_free_universe(h)
             self._h = None
 
 
@@ -214,9 +217,12 @@ class Plan:
         return out
 
     def __del__(self):
+        # At interpreter exit the module globals may already be cleared (the process releases
+        # the device memory itself then).
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            nat.lib.sdesim_plan_free(h)
+        lib = getattr(nat, "lib", None)
+        if h is not None and h.value and lib is not None:
+            lib.sdesim_plan_free(h)
             self._h = None
 
 

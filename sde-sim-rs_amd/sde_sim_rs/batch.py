@@ -68,7 +68,10 @@ class Batch:
         return out
 
     def __del__(self):
+        # At interpreter exit the module globals may already be cleared (the process releases
+        # the device memory itself then).
         h = getattr(self, "_h", None)
-        if h is not None and h.value:
-            nat.lib.sdesim_batch_free(h)
+        lib = getattr(nat, "lib", None)
+        if h is not None and h.value and lib is not None:
+            lib.sdesim_batch_free(h)
             self._h = None
