@@ -283,3 +283,47 @@ def test_long_layout_transpose(model, T, precision, S):
     want = dev.cpu().numpy().astype(np.float64).transpose(2, 0, 1)
     assert got.dtype == np.float64 and got.shape == want.shape
     np.testing.assert_array_equal(got, want)
+
+
+def test_concurrent_plans_from_threads():
+    """Plans created and run from several host threads at once (the GIL is released in every
+    native call; the compiled-module cache is shared and locked) give the results of the same
+    plans run one after another. Each thread uses its own stream."""
+    import threading
+    import sde_sim_rs
+    times = np.linspace(0.0, 1.0, 33)
+    cases = [(GBM, {"X": 100.0}, "pseudo", "euler", s) for s in (1, 2)] + \
+            [(HESTON, {"S": 100.0, "v": 0.04}, "sobol", "runge-kutta", s) for s in (3, 4)]
+    print("case: %d concurrent plans" % len(cases), flush=True)
+
+    def run(case, stream=None):
+        eqs, init, rng, scheme, seed = case
+        plan = sde_sim_rs.Plan(eqs, times, 2000, init, rng, scheme, seed=seed, scramble="xor", scramble_seed=seed)
+        f = plan.new_filtration()
+        if stream is None:
+            plan.run(f)
+        else:
+            with torch.cuda.stream(stream):
+                plan.run(f, stream=stream)
+            stream.synchronize()
+        torch.cuda.synchronize()
+        return f.cpu().numpy()
+
+    want = [run(c) for c in cases]
+    got = [None] * len(cases)
+    errors = []
+
+    def worker(i):
+        try:
+            got[i] = run(cases[i], torch.cuda.Stream())
+        except Exception as e:  # reported below
+            errors.append(e)
+
+    threads = [threading.Thread(target=worker, args=(i,)) for i in range(len(cases))]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g, w)
