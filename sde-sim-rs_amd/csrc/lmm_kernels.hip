@@ -70,16 +70,17 @@ __device__ __forceinline__ void lmm_noise(__amdgpu_buffer_rsrc_t rsrc, const Lmm
 }
 
 // Model parameters: the workgroup stages them once from the device buffer ([40] volatilities
-// then [40][3] loadings) into LDS, one row per forward with the step's constant products
+// then [40][3] loadings) into LDS, one 8-value row per forward with the step's constant products
 // precomputed, so a forward's parameters are two broadcast LDS reads. Each step reads them at
 // index i + off, where off = step * a.zero is 0 at run time but not known to the compiler: the
 // reads repeat per step (issued ahead of use within the step) instead of 320 loop-invariant
 // values being hoisted into registers for the whole kernel.
 template <typename R>
 struct LmmParRow {
-  R sig, ts;      // sigma_i, tau sigma_i
+  R ts;           // tau sigma_i
   R b0, b1, b2;   // loadings B_i0..B_i2
-  R sd, ss, h;    // exponent terms sigma dt, sigma sqrt(dt), -sigma^2 dt / 2 (f32: times log2 e)
+  R q0, q1, q2;   // sigma_i B_i0..B_i2
+  R h;            // -sigma_i^2 dt / 2 (f32: times log2 e)
 };
 
 template <typename R>
@@ -95,24 +96,25 @@ __device__ __forceinline__ R lmm_recip(R x) {
   else return R(1) / x;
 }
 
-// One log-Euler step of all alive forwards (forwards below `eta` are dead and unchanged).
+// One log-Euler step of all alive forwards (forwards below `eta` are dead and unchanged). The
+// exponent (sigma_i (B_i.C) - sigma_i^2 / 2) dt + sigma_i sqrt(dt) (B_i.eps) is formed as
+// q_i.(dt C + sqrt(dt) eps) + h_i with q_i = sigma_i B_i: the step's sqrt(dt) eps is shared by all
+// forwards. dtu and sdtu are dt and sqrt(dt) in the exponent's unit (log2 e for f32).
 template <typename R>
-__device__ __forceinline__ void lmm_step(const LmmPar<R>& tab, R (&L)[NF], const R (&eps)[F], int eta, R dt, R sdt,
+__device__ __forceinline__ void lmm_step(const LmmPar<R>& tab, R (&L)[NF], const R (&eps)[F], int eta, R dtu, R sdtu,
                                          R tau) {
   R C[F] = {R(0), R(0), R(0)};
+  const R se0 = sdtu * eps[0], se1 = sdtu * eps[1], se2 = sdtu * eps[2];
 #pragma unroll
   for (int i = 0; i < NF; ++i) {
     if (i >= eta) {
       const LmmParRow<R> pr = tab.row(i);
-      const R b0 = pr.b0, b1 = pr.b1, b2 = pr.b2;
       const R ai = pr.ts * L[i] * lmm_recip<R>(R(1) + tau * L[i]);
-      C[0] = C[0] + ai * b0;
-      C[1] = C[1] + ai * b1;
-      C[2] = C[2] + ai * b2;
-      // (sigma (B.C) - sigma^2 / 2) dt + sigma sqrt(dt) (B.eps)
-      const R bc = b0 * C[0] + b1 * C[1] + b2 * C[2];
-      const R w = b0 * eps[0] + b1 * eps[1] + b2 * eps[2];
-      L[i] = L[i] * lmm_exp_row<R>(pr.sd * bc + pr.ss * w + pr.h);
+      C[0] = C[0] + ai * pr.b0;
+      C[1] = C[1] + ai * pr.b1;
+      C[2] = C[2] + ai * pr.b2;
+      const R x = pr.q0 * (dtu * C[0] + se0) + pr.q1 * (dtu * C[1] + se1) + pr.q2 * (dtu * C[2] + se2) + pr.h;
+      L[i] = L[i] * lmm_exp_row<R>(x);
     }
   }
 }
@@ -123,11 +125,11 @@ __device__ __forceinline__ void lmm_step(const LmmPar<R>& tab, R (&L)[NF], const
 template <typename R>
 __device__ __forceinline__ void lmm_forward_adjoint(const LmmParRow<R>& pr, R Li, R Lpi, R& Lbi, R (&G)[F], R dt,
                                                     R tau) {
-  const R sig = pr.sig, b0 = pr.b0, b1 = pr.b1, b2 = pr.b2;
+  const R b0 = pr.b0, b1 = pr.b1, b2 = pr.b2;
   const R db = Lbi * Lpi * dt;
-  G[0] = G[0] + db * sig * b0;
-  G[1] = G[1] + db * sig * b1;
-  G[2] = G[2] + db * sig * b2;
+  G[0] = G[0] + db * pr.q0;
+  G[1] = G[1] + db * pr.q1;
+  G[2] = G[2] + db * pr.q2;
   const R ab = b0 * G[0] + b1 * G[1] + b2 * G[2];
   const R rop = lmm_recip<R>(R(1) + tau * Li);
   Lbi = Lbi * (Lpi * lmm_recip<R>(Li)) + ab * pr.ts * (rop * rop);
@@ -187,13 +189,13 @@ __device__ __forceinline__ R lmm_payoff(const R (&L)[NF], int e, R tau, R K, R (
 template <typename R>
 __device__ __forceinline__ void lmm_stage_params(const LmmArgs& a, LmmParRow<R>* par_rows) {
   if (threadIdx.x < NF) {
-    const R dt = (R)a.dt, sdt = (R)a.sdt, tau = (R)a.tau;
+    const R dt = (R)a.dt, tau = (R)a.tau;
     const R* par = sizeof(R) == 4 ? (const R*)(const void*)a.parf : (const R*)(const void*)a.pard;
     const int i = (int)threadIdx.x;
     const R sig = par[i];
     const R un = sizeof(R) == 4 ? R(1.4426950408889634) : R(1);  // exponent unit: log2 e for f32
-    par_rows[i] = LmmParRow<R>{sig, tau * sig, par[NF + i * F], par[NF + i * F + 1], par[NF + i * F + 2],
-                               sig * dt * un, sig * sdt * un, R(-0.5) * sig * sig * dt * un};
+    const R b0 = par[NF + i * F], b1 = par[NF + i * F + 1], b2 = par[NF + i * F + 2];
+    par_rows[i] = LmmParRow<R>{tau * sig, b0, b1, b2, sig * b0, sig * b1, sig * b2, R(-0.5) * sig * sig * dt * un};
   }
   __syncthreads();
 }
@@ -213,7 +215,9 @@ sdesim_lmm(const LmmArgs a) {
   const bool live = s < S;
   const long long sid = live ? s : 0;
   const sde_u32 voff = live ? (sde_u32)(sid * (long long)sizeof(R)) : 0x80000000u;
-  const R dt = (R)a.dt, sdt = (R)a.sdt, tau = (R)a.tau, K = (R)a.strike;
+  const R dt = (R)a.dt, tau = (R)a.tau, K = (R)a.strike;
+  const R un = sizeof(R) == 4 ? R(1.4426950408889634) : R(1);  // exponent unit: log2 e for f32
+  const R dtu = (R)a.dt * un, sdtu = (R)a.sdt * un;
   const int e = a.expiry_index;
 
   __shared__ LmmParRow<R> par_rows[NF];
@@ -248,7 +252,7 @@ sdesim_lmm(const LmmArgs a) {
     }
     R eps[F];
     lmm_noise<R>(rsrc, sb, n, eps);
-    lmm_step<R>(LmmPar<R>{par_rows, n * a.zero}, L, eps, n / a.steps_per_period + 1, dt, sdt, tau);
+    lmm_step<R>(LmmPar<R>{par_rows, n * a.zero}, L, eps, n / a.steps_per_period + 1, dtu, sdtu, tau);
   }
   if (MODE == 2) {
     sde_tape_store<R, NF>(a.tape, N, S, voff, L);
@@ -278,7 +282,7 @@ sdesim_lmm(const LmmArgs a) {
         }
         R eps[F];
         lmm_noise<R>(rsrc, sb, n0 + k, eps);
-        lmm_step<R>(LmmPar<R>{par_rows, (n0 + k) * a.zero}, Xc, eps, (n0 + k) / a.steps_per_period + 1, dt, sdt,
+        lmm_step<R>(LmmPar<R>{par_rows, (n0 + k) * a.zero}, Xc, eps, (n0 + k) / a.steps_per_period + 1, dtu, sdtu,
                     tau);
       }
       for (int k = len - 1; k >= 0; --k) {
