@@ -327,3 +327,32 @@ def test_concurrent_plans_from_threads():
     assert not errors, errors
     for g, w in zip(got, want):
         np.testing.assert_array_equal(g, w)
+
+
+def test_plan_run_captured_in_a_graph():
+    """Plan.run issues launches only (no allocation, memset or synchronisation), so a caller can
+    capture it into a graph: the config-1 shape (10,000 paths x 252 steps, launch-bound) captured
+    with torch.cuda.graph, replayed three times with eager work and synchronisation in between,
+    gives the eager filtration and moments on every replay."""
+    import sde_sim_rs
+    times = np.linspace(0.0, 1.0, 253)
+    print("case: graph capture of Plan.run (filtration + terminal)", flush=True)
+    plan = sde_sim_rs.Plan(GBM, times, 10000, {"X": 100.0}, "pseudo", "euler", seed=42,
+                           outputs=("filtration", "terminal"))
+    f = plan.new_filtration()
+    plan.run(f)
+    torch.cuda.synchronize()
+    want_f = f.cpu().numpy()
+    want_m = plan.moments()
+    g = torch.cuda.CUDAGraph()
+    f.zero_()
+    torch.cuda.synchronize()
+    with torch.cuda.graph(g):
+        plan.run(f)
+    for rep in range(3):
+        f.fill_(-1.0)  # eager work between replays
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(f.cpu().numpy(), want_f, err_msg="replay %d" % rep)
+        assert plan.moments() == want_m, rep
