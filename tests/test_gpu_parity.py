@@ -356,3 +356,51 @@ def test_plan_run_captured_in_a_graph():
         torch.cuda.synchronize()
         np.testing.assert_array_equal(f.cpu().numpy(), want_f, err_msg="replay %d" % rep)
         assert plan.moments() == want_m, rep
+
+
+def _random_coef(rng, names, depth):
+    """A random continuous expression in the DSL's expression language over the process names and
+    t: sums, differences, products, guarded quotients and powers, min/max, abs and bounded
+    transcendental functions (no comparisons or rounding, whose discontinuities would turn
+    last-ulp differences between the two libms into different paths)."""
+    if depth == 0 or rng.random() < 0.25:
+        c = rng.integers(0, 4)
+        if c == 0:
+            return str(rng.choice(names))
+        if c == 1:
+            return "t"
+        return "%.4g" % rng.uniform(0.1, 2.0)
+    a = _random_coef(rng, names, depth - 1)
+    b = _random_coef(rng, names, depth - 1)
+    k = rng.integers(0, 12)
+    return ["(%s + %s)" % (a, b), "(%s - %s)" % (a, b), "(%s * %s)" % (a, b), "(%s / (1 + abs(%s)))" % (a, b),
+            "(abs(%s) ^ 0.5)" % a, "min(%s, %s)" % (a, b), "max(%s, %s, 0.3)" % (a, b), "sin(%s)" % a,
+            "cos(%s)" % a, "tanh(%s)" % a, "log(1 + abs(%s))" % a, "atan(%s)" % a][k]
+
+
+def test_random_universes_match_oracle(oracle):
+    """Random equation sets through the code generator and the device kernel against the oracle's
+    interpreter: 1-2 processes, 1-2 Wiener increments per process, random bounded coefficient
+    expressions, both schemes, pseudo / Sobol-shift / Sobol-XOR, FP64, 17 time points, 300 paths.
+    The bar allows for last-ulp differences between the device's and the host's libm."""
+    rng = np.random.default_rng(21)
+    times = np.linspace(0.0, 1.0, 17)
+    for case in range(16):
+        P = int(rng.integers(1, 3))
+        names = ["X", "Y"][:P]
+        eqs = []
+        for n in names:
+            terms = ["(0.1 * sin(%s)) * dt" % _random_coef(rng, names, 3)]
+            for k in range(int(rng.integers(1, 3))):
+                terms.append("(0.2 * cos(%s) * %s) * dW%d" % (_random_coef(rng, names, 3), n, k + 1))
+            eqs.append("d%s = %s" % (n, " + ".join(terms)))
+        scheme = str(rng.choice(["euler", "runge-kutta"]))
+        rng_method, scramble = [("pseudo", "shift"), ("sobol", "shift"), ("sobol", "xor")][int(rng.integers(0, 3))]
+        init = {n: float(rng.uniform(0.5, 1.5)) for n in names}
+        seed = int(rng.integers(0, 1000))
+        print("case %d: %s %s/%s %s" % (case, scheme, rng_method, scramble, eqs), flush=True)
+        _, f = _filtration(eqs, times, 300, init, rng_method, scheme, seed=seed, scramble=scramble,
+                           scramble_seed=seed + 1)
+        ref = oracle.simulate(eqs, times, 300, init, scheme=scheme, rng=rng_method, seed=seed, scramble=scramble,
+                              scramble_seed=seed + 1)
+        _close(f.transpose(2, 0, 1), ref, rtol=1e-10, atol=1e-12)
