@@ -128,6 +128,14 @@ int sdesim_generate_source(const sdesim_universe* u, int32_t n_times, const sdes
                            char* buf, int64_t buf_len, int64_t* needed);
 int sdesim_compile_source(const char* source, int64_t* code_bytes);
 
+/* Compiled run-time kernels are kept per process (by device and source) and on disk in the cache
+ * directory (SDESIM_CACHE_DIR, else $HOME/.cache/sdesim), so a new process loads a kernel it has
+ * compiled before instead of compiling it again. No reference counterpart (the reference
+ * compiles nothing at run time). sdesim_kernel_cache_reset forgets the per-process table: the
+ * next plan of a given source loads it from disk or compiles it. Modules already loaded stay
+ * loaded until the process ends, so existing plans keep working. */
+void sdesim_kernel_cache_reset(void);
+
 /* Host evaluation of one coefficient expression with the kernels' semantics (the value
  * Function::eval, src/func.rs:32-42, returns for that namespace). An undefined name returns
  * SDESIM_EPANIC, as the reference's unwrap does. */

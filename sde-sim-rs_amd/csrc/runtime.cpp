@@ -9,10 +9,13 @@
 #include <hip/hip_runtime_api.h>
 #include <hip/hiprtc.h>
 
+#include <unistd.h>
+
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <filesystem>
 #include <fstream>
 #include <map>
 #include <sstream>
@@ -93,12 +96,13 @@ struct CompiledKernel {
 std::mutex g_cache_mu;
 std::map<std::pair<int, std::string>, CompiledKernel> g_kernels;
 
-// Compile generated source to a gfx950 code object (needs no device).
-std::vector<char> rtc_compile(const std::string& src) {
-  hiprtcProgram prog;
-  // Kernel development: SDESIM_DEVICE_HEADER names a device header file to compile run-time
-  // kernels against instead of the copy embedded at build time.
-  static const std::string dev_header = [] {
+const char* const kRtcOptions[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast",
+                                   "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
+
+// The device header the run-time kernels compile against. Kernel development:
+// SDESIM_DEVICE_HEADER names a header file to use instead of the copy embedded at build time.
+const std::string& device_header_text() {
+  static const std::string text = [] {
     const char* path = std::getenv("SDESIM_DEVICE_HEADER");
     if (!path || !*path) return std::string(sdesim_embedded_device_header);
     std::ifstream in(path);
@@ -107,13 +111,17 @@ std::vector<char> rtc_compile(const std::string& src) {
     ss << in.rdbuf();
     return ss.str();
   }();
-  const char* hdrs[] = {dev_header.c_str(), sdesim_embedded_args_header};
+  return text;
+}
+
+// Compile generated source to a gfx950 code object (needs no device).
+std::vector<char> rtc_compile(const std::string& src) {
+  hiprtcProgram prog;
+  const char* hdrs[] = {device_header_text().c_str(), sdesim_embedded_args_header};
   const char* names[] = {"sdesim_device.hpp", "sdesim_args.hpp"};
   if (hiprtcCreateProgram(&prog, src.c_str(), "sdesim_kernel.hip", 2, hdrs, names) != HIPRTC_SUCCESS)
     fail(SDESIM_ERUNTIME, "hiprtcCreateProgram failed");
-  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast",
-                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
-  hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(opts) / sizeof(opts[0])), opts);
+  hiprtcResult r = hiprtcCompileProgram(prog, (int)(sizeof(kRtcOptions) / sizeof(kRtcOptions[0])), kRtcOptions);
   if (r != HIPRTC_SUCCESS) {
     size_t n = 0;
     hiprtcGetProgramLogSize(prog, &n);
@@ -130,6 +138,65 @@ std::vector<char> rtc_compile(const std::string& src) {
   return code;
 }
 
+// Code objects persist across processes in the cache directory (SDESIM_CACHE_DIR, else
+// $HOME/.cache/sdesim), one file per kernel named by a 64-bit FNV-1a hash of everything the code
+// depends on: the generated source, both device headers, the compile options and the hipRTC
+// version. A file holds a 24-byte header (magic, code size, FNV-1a of the code) before the code
+// object; one that is missing, short or fails the check is compiled again, before anything is
+// handed to the HIP loader. New files are written under a temporary name and renamed into place,
+// so concurrent processes never read a partial file. Writing is best effort: a read-only or
+// missing home directory only costs the compile.
+uint64_t fnv1a(const void* data, size_t n, uint64_t h = 0xcbf29ce484222325ull) {
+  const unsigned char* p = (const unsigned char*)data;
+  for (size_t i = 0; i < n; ++i) h = (h ^ p[i]) * 0x100000001b3ull;
+  return h;
+}
+
+constexpr uint64_t kCodeMagic = 0x314b4f43454453ull;  // "SDECOK1"
+
+std::string code_cache_file(const std::string& src) {
+  const char* dir = std::getenv("SDESIM_CACHE_DIR");
+  std::string d;
+  if (dir && *dir) d = dir;
+  else if (const char* home = std::getenv("HOME")) d = std::string(home) + "/.cache/sdesim";
+  else return std::string();
+  int major = 0, minor = 0;
+  hiprtcVersion(&major, &minor);
+  uint64_t h = fnv1a(src.data(), src.size());
+  const std::string& hdr = device_header_text();
+  h = fnv1a(hdr.data(), hdr.size(), h);
+  h = fnv1a(sdesim_embedded_args_header, std::strlen(sdesim_embedded_args_header), h);
+  for (const char* o : kRtcOptions) h = fnv1a(o, std::strlen(o) + 1, h);
+  const int ver[2] = {major, minor};
+  h = fnv1a(ver, sizeof ver, h);
+  char name[40];
+  std::snprintf(name, sizeof name, "/k_%016llx.co", (unsigned long long)h);
+  return d + name;
+}
+
+std::vector<char> read_cached_code(const std::string& path) {
+  std::ifstream in(path, std::ios::binary);
+  uint64_t head[3] = {0, 0, 0};
+  if (!in || !in.read((char*)head, sizeof head) || head[0] != kCodeMagic || head[1] == 0 || head[1] > (1ull << 30))
+    return {};
+  std::vector<char> code(head[1]);
+  if (!in.read(code.data(), (std::streamsize)code.size()) || fnv1a(code.data(), code.size()) != head[2]) return {};
+  return code;
+}
+
+void write_cached_code(const std::string& path, const std::vector<char>& code) {
+  std::error_code ec;
+  std::filesystem::create_directories(std::filesystem::path(path).parent_path(), ec);
+  const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+  {
+    std::ofstream out(tmp, std::ios::binary | std::ios::trunc);
+    const uint64_t head[3] = {kCodeMagic, (uint64_t)code.size(), fnv1a(code.data(), code.size())};
+    if (!out || !out.write((const char*)head, sizeof head) || !out.write(code.data(), (std::streamsize)code.size()))
+      ec = std::make_error_code(std::errc::io_error);
+  }
+  if (ec || std::rename(tmp.c_str(), path.c_str()) != 0) std::filesystem::remove(tmp, ec);
+}
+
 CompiledKernel compile_kernel(const std::string& src, const std::string& name) {
   int dev = 0;
   hip_check(hipGetDevice(&dev), "hipGetDevice");
@@ -137,9 +204,21 @@ CompiledKernel compile_kernel(const std::string& src, const std::string& name) {
   auto key = std::make_pair(dev, src);
   auto it = g_kernels.find(key);
   if (it != g_kernels.end()) return it->second;
-  std::vector<char> code = rtc_compile(src);
+  const std::string path = code_cache_file(src);
   CompiledKernel k;
-  hip_check(hipModuleLoadData(&k.module, code.data()), "hipModuleLoadData");
+  bool loaded = false;
+  if (!path.empty()) {
+    const std::vector<char> code = read_cached_code(path);
+    if (!code.empty()) {
+      loaded = hipModuleLoadData(&k.module, code.data()) == hipSuccess;
+      if (!loaded) (void)hipGetLastError();  // a code object the runtime refuses: compile instead
+    }
+  }
+  if (!loaded) {
+    const std::vector<char> code = rtc_compile(src);
+    hip_check(hipModuleLoadData(&k.module, code.data()), "hipModuleLoadData");
+    if (!path.empty()) write_cached_code(path, code);
+  }
   hip_check(hipModuleGetFunction(&k.function, k.module, name.c_str()), "hipModuleGetFunction");
   g_kernels[key] = k;
   return k;
@@ -664,6 +743,11 @@ int sdesim_generate_source(const sdesim_universe* u, int32_t n_times, const sdes
     if (needed) *needed = (int64_t)src.size() + 1;
     if (buf && buf_len > 0) std::snprintf(buf, (size_t)buf_len, "%s", src.c_str());
   });
+}
+
+void sdesim_kernel_cache_reset(void) {
+  std::lock_guard<std::mutex> lock(g_cache_mu);
+  g_kernels.clear();
 }
 
 int sdesim_compile_source(const char* source, int64_t* code_bytes) {

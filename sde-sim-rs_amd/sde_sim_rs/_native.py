@@ -129,6 +129,7 @@ SIGNATURES = {
                                          ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                          ctypes.POINTER(ctypes.c_int64)]),
     "sdesim_basket_create": (ctypes.c_int, [ctypes.POINTER(BasketSpec), ctypes.POINTER(ctypes.c_void_p)]),
+    "sdesim_kernel_cache_reset": (None, []),
     "sdesim_basket_free": (None, [ctypes.c_void_p]),
     "sdesim_basket_storage": (ctypes.c_int32, [ctypes.c_void_p]),
     "sdesim_basket_eval": (ctypes.c_int, [ctypes.c_void_p] + [ctypes.POINTER(ctypes.c_double)] * 5 +

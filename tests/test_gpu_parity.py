@@ -404,3 +404,38 @@ def test_random_universes_match_oracle(oracle):
         ref = oracle.simulate(eqs, times, 300, init, scheme=scheme, rng=rng_method, seed=seed, scramble=scramble,
                               scramble_seed=seed + 1)
         _close(f.transpose(2, 0, 1), ref, rtol=1e-10, atol=1e-12)
+
+
+def test_kernel_code_cache_on_disk(tmp_path, monkeypatch):
+    """A compiled kernel is written to the cache directory and loaded from it once the in-process
+    table is forgotten (as by a new process); a damaged file is rejected by its checksum before
+    the loader sees it, compiled again and rewritten. The results are the same each time."""
+    import os
+    import sde_sim_rs
+    from sde_sim_rs import _native as nat
+    monkeypatch.setenv("SDESIM_CACHE_DIR", str(tmp_path))
+    eqs = ["dX = (0.0371 * X) * dt + (0.19 * X) * dW1"]  # a source no other test compiles
+    times = np.linspace(0.0, 1.0, 9)
+    print("case: kernel code cache in %s" % tmp_path, flush=True)
+
+    def run():
+        _, f = _filtration(eqs, times, 1000, {"X": 100.0}, "pseudo", "euler", seed=7)
+        return f
+
+    nat.lib.sdesim_kernel_cache_reset()
+    f1 = run()
+    files = sorted(p for p in os.listdir(tmp_path) if p.endswith(".co"))
+    assert len(files) == 1, os.listdir(tmp_path)
+    path = os.path.join(tmp_path, files[0])
+    blob = open(path, "rb").read()
+    assert len(blob) > 24
+    nat.lib.sdesim_kernel_cache_reset()
+    np.testing.assert_array_equal(run(), f1)
+    assert open(path, "rb").read() == blob  # loaded, not rewritten
+    damaged = blob[:24] + bytes(len(blob) - 24)
+    open(path, "wb").write(damaged)
+    nat.lib.sdesim_kernel_cache_reset()
+    np.testing.assert_array_equal(run(), f1)
+    again = open(path, "rb").read()  # recompiled and rewritten
+    assert again != damaged and again[:8] == blob[:8] and len(again) > 24
+    assert sorted(p for p in os.listdir(tmp_path)) == files  # no temporary files left behind
