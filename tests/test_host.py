@@ -371,3 +371,14 @@ def test_long_table_layout_and_chunks(chunk_rows):
             for pi, n in enumerate(names):
                 assert (sc[k], tm[k], pn[k], vv[k]) == (s, t, n, values[s, ti, pi])
                 k += 1
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus N > 1 exits with status 2 and says why, before touching
+    a device."""
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in r.stderr
