@@ -155,3 +155,31 @@ def test_adjoint_tape_equals_checkpoints(precision, D):
     for name in ("x0", "mu", "sigma", "chol"):
         scale = np.max(np.abs(k[name]))
         np.testing.assert_allclose(t[name], k[name], rtol=0, atol=rtol * scale, err_msg=name)
+
+
+def test_handles_release_device_memory():
+    """Creating and freeing adjoint handles returns their device memory (the basket's state tape
+    here is 3.4 GB): after one cycle (which also lets the runtime make its one-time allocations,
+    such as loading the kernels' code), nine more create/evaluate/free cycles leave free memory
+    where it was."""
+    import gc
+    from sde_sim_rs.adjoint import Basket
+    from sde_sim_rs.lmm import ForwardRateModel, forward_rate_config
+    c, lc = _cfg(), forward_rate_config(seed=5)
+    print("case: handle memory release", flush=True)
+
+    def cycle():
+        b = Basket(8, 100, 1 << 20, seed=1, precision="f32", storage="tape")
+        b.evaluate(c["x0"], c["mu"], c["sigma"], c["rho"], adjoint=True)
+        m = ForwardRateModel(steps=40, paths=1 << 14, precision="f32", storage="checkpoints")
+        m.evaluate(lc["L0"], lc["vol"], lc["loadings"], adjoint=True)
+        del b, m
+        gc.collect()
+        torch.cuda.synchronize()
+
+    cycle()
+    before = torch.cuda.mem_get_info()[0]
+    for _ in range(9):
+        cycle()
+    after = torch.cuda.mem_get_info()[0]
+    assert after >= before - (16 << 20), (before, after)
