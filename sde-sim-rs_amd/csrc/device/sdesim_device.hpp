@@ -405,9 +405,13 @@ struct SdeChunk {
 
 typedef sde_u32 SdeU32x2 __attribute__((ext_vector_type(2)));
 
-template <int RNG> struct SdeUnifOf { typedef double type; };
-template <> struct SdeUnifOf<SDE_RNG_SOBOL_XOR> { typedef SdeUnifX type; };
-template <> struct SdeUnifOf<SDE_RNG_PSEUDO> { typedef SdeUnifW type; };
+// The uniform a step consumes: pseudo-random draws (and, in FP32, Sobol points with the random
+// shift) as two 32-bit words of a 53-bit fraction, XOR-scrambled Sobol points as their word,
+// otherwise an f64 value.
+template <int RNG, typename R> struct SdeUnifOf { typedef double type; };
+template <typename R> struct SdeUnifOf<SDE_RNG_SOBOL_XOR, R> { typedef SdeUnifX type; };
+template <typename R> struct SdeUnifOf<SDE_RNG_PSEUDO, R> { typedef SdeUnifW type; };
+template <> struct SdeUnifOf<SDE_RNG_SOBOL_SHIFT, float> { typedef SdeUnifW type; };
 
 template <typename R>
 struct SdeGrid {
@@ -466,7 +470,7 @@ __device__ __forceinline__ void sde_load_words(__amdgpu_buffer_rsrc_t r, int vof
 template <class Mdl, int RNG, int OUT, int PPL, int NSL, bool UG, bool AL = false>
 __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
   typedef typename Mdl::real real;
-  typedef typename SdeUnifOf<RNG>::type unif;
+  typedef typename SdeUnifOf<RNG, real>::type unif;
   constexpr int P = Mdl::P;
   constexpr int M = Mdl::M;
   constexpr int MM = M > 0 ? M : 1;
@@ -673,6 +677,15 @@ __device__ __forceinline__ void sde_path_kernel(const SimArgs& a) {
               const int w = 2 * (i * M + k);
               u[j][k].lo = W[j][USE_CHACHA ? w : 0];
               u[j][k].hi = W[j][USE_CHACHA ? w + 1 : 0];
+            } else if constexpr (RNG == SDE_RNG_SOBOL_SHIFT && sizeof(real) == 4) {
+              // frac(y 2^-32 + shift) as an exact 53-bit fraction: (y 2^21 + (word >> 11)) mod
+              // 2^53, held as the 64-bit word (fraction << 11). The f64 path rounds the sum to
+              // 53 bits first when it exceeds 1, a difference of at most 2^-53 in the uniform.
+              const int w = 2 * (i * M + k);
+              const sde_u64 sh = (((sde_u64)W[j][USE_CHACHA ? w + 1 : 0] << 32) | W[j][USE_CHACHA ? w : 0]) >> 11;
+              const sde_u64 f = (((sde_u64)nx[j][k] << 21) + sh) << 11;
+              u[j][k].lo = (sde_u32)f;
+              u[j][k].hi = (sde_u32)(f >> 32);
             } else {
               u[j][k] = v;
             }

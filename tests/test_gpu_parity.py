@@ -162,16 +162,18 @@ def test_fp32_xor_tail_increments(oracle):
     np.testing.assert_allclose(out["f32"], out["f64"], rtol=0, atol=2e-4)
 
 
+@pytest.mark.parametrize("rng", ["pseudo", "sobol"])
 @pytest.mark.parametrize("scheme", ["euler", "runge-kutta"])
-def test_fp32_pseudo_matches_oracle(oracle, scheme):
-    """FP32 pseudo-random paths (the inverse normal formed from each draw's two ChaCha words, the
-    RK sign from the words) follow the oracle's FP64 paths to FP32 accuracy: GBM, 252 steps,
-    5000 paths (a partial last block), every value within 5e-5 relative."""
+def test_fp32_pseudo_matches_oracle(oracle, scheme, rng):
+    """FP32 pseudo-random and Sobol-shift paths (the uniform held as a 53-bit fraction in two
+    words: the ChaCha draw, or the Sobol point plus the draw's shift modulo 1; the inverse normal
+    and the RK sign formed from the words) follow the oracle's FP64 paths to FP32 accuracy: GBM,
+    252 steps, 5000 paths (a partial last block), every value within 5e-5 relative."""
     times = np.linspace(0.0, 1.0, 253)
     S = 5000
-    print("case: fp32 pseudo %s gbm S=%d N=252" % (scheme, S), flush=True)
-    _, f = _filtration(GBM, times, S, {"X": 100.0}, "pseudo", scheme, seed=42, precision="f32")
-    ref = oracle.simulate(GBM, times, S, {"X": 100.0}, scheme=scheme, rng="pseudo", seed=42)
+    print("case: fp32 %s %s gbm S=%d N=252" % (rng, scheme, S), flush=True)
+    _, f = _filtration(GBM, times, S, {"X": 100.0}, rng, scheme, seed=42, precision="f32")
+    ref = oracle.simulate(GBM, times, S, {"X": 100.0}, scheme=scheme, rng=rng, seed=42)
     _close(f.transpose(2, 0, 1), ref, rtol=5e-5, atol=0.0)
 
 

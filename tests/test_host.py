@@ -202,7 +202,7 @@ def generate(eqs, n_times, payoff=None, **opts):
 @pytest.mark.parametrize("eqs", [GBM, HESTON, JUMP], ids=["gbm", "heston", "jump"])
 @pytest.mark.parametrize("scheme", [0, 1])
 @pytest.mark.parametrize("precision", [0, 1])
-@pytest.mark.parametrize("rng", [(0, 0), (1, 1)], ids=["pseudo", "sobolxor"])
+@pytest.mark.parametrize("rng", [(0, 0), (1, 0), (1, 1)], ids=["pseudo", "sobolshift", "sobolxor"])
 def test_generated_kernels_compile(eqs, scheme, precision, rng):
     src = generate(eqs, 33, payoff=None, scheme=scheme, precision=precision, rng=rng[0], scramble=rng[1],
                    outputs=3 if precision else 1)
