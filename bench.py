@@ -56,6 +56,17 @@ def _time_plan(plan, out, steps, warmup):
     return wall, kern
 
 
+def _time_eval(fn, args):
+    """Mean wall ms of a synchronous evaluation after max(1, warmup) untimed ones."""
+    for _ in range(max(1, args.warmup)):
+        fn()
+    reps = max(1, args.steps // 4)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        fn()
+    return (time.perf_counter() - t0) * 1e3 / reps
+
+
 def _time_fill(buf, steps, warmup):
     """Mean ms of a plain fill of the same buffer (torch's elementwise fill kernel): the write
     rate this many bytes reach on this GPU without any computation in the loop."""
@@ -242,6 +253,13 @@ def main():
                "storage": bk.storage,
                "state_bytes": (AN + 1) * 8 * AS * 4 if bk.storage == "tape" else ((AN + 18) // 19) * 8 * AS * 4}
         del bk
+        if adj["storage"] == "tape":
+            # The checkpoint form (every 19th state, the rest recomputed in the reverse sweep),
+            # timed the same way for comparison.
+            bk = Basket(8, AN, AS, horizon=1.0, strike=100.0, width=0.5, seed=3, precision="f32", storage="checkpoints")
+            adj["adjoint_ms_checkpoints"] = _time_eval(
+                lambda: bk.evaluate(c["x0"], c["mu"], c["sigma"], c["rho"], adjoint=True), args)
+            del bk
 
     lmm = None
     if "lmm" in legs:
@@ -265,6 +283,12 @@ def main():
                "forward_steps_per_s_price": LS * LN * 40 / (times[False] * 1e-3), "price": r["price"],
                "storage": fm.storage}
         del fm
+        if lmm["storage"] == "tape":
+            fm = ForwardRateModel(steps=LN, paths=LS, expiry_index=4, strike=0.03, scramble_seed=5, precision="f32",
+                                  storage="checkpoints")
+            lmm["adjoint_ms_checkpoints"] = _time_eval(
+                lambda: fm.evaluate(c["L0"], c["vol"], c["loadings"], adjoint=True), args)
+            del fm
 
     path_steps = S * N
     line = {"metric": "path-steps/s on 1 MI355X, FP32 vs FP64 plain at equal error; HBM GB/s %peak",
