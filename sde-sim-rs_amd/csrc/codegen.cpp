@@ -354,11 +354,7 @@ std::string generate_kernel_source(const Universe& u, const CodegenOptions& o) {
   }
   s << "  }\n";
   s << "};\n\n";
-  // Kernel development: SDESIM_WAVES_PER_EU asks the compiler for at least that many waves per
-  // SIMD (it then limits registers accordingly).
-  const char* wpe = std::getenv("SDESIM_WAVES_PER_EU");
-  std::string attr = (wpe && std::atoi(wpe) > 0) ? " __attribute__((amdgpu_waves_per_eu(" + std::to_string(std::atoi(wpe)) + ")))" : "";
-  s << "extern \"C\" __global__ void __launch_bounds__(" << o.block << ")" << attr << " " << o.name << "(SimArgs a) {\n";
+  s << "extern \"C\" __global__ void __launch_bounds__(" << o.block << ") " << o.name << "(SimArgs a) {\n";
   s << "  sde_path_kernel<SdeModel, " << o.rng << ", " << o.out << ", " << o.ppl << ", " << o.nsl << ", "
     << (o.uniform_grid ? "true" : "false") << ", " << (o.wave_aligned ? "true" : "false") << ">(a);\n";
   s << "}\n";
