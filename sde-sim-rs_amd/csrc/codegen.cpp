@@ -16,7 +16,6 @@
 // the reference computes twice from the same cache and time (e.g. RK stage-2 coefficients at
 // t_{n+1} and the next step's stage-1 coefficients at the same time on the same probe cache)
 // are carried in registers across steps instead of being recomputed.
-#include <cstdlib>
 #include <map>
 #include <set>
 #include <sstream>
