@@ -85,10 +85,19 @@ struct LmmParRow {
 
 template <typename R>
 struct LmmPar {
-  const LmmParRow<R>* rows;  // LDS
-  int off;
-  __device__ __forceinline__ LmmParRow<R> row(int i) const { return rows[i + off]; }
+  const LmmParRow<R>* rows;  // LDS, already advanced by the step's offset
+  __device__ __forceinline__ LmmParRow<R> row(int i) const { return rows[i]; }
 };
+
+// The step's view of the parameter rows: rows + off, with the offset moved into a vector register
+// the compiler cannot see through, so the LDS address is formed once per step and each forward's
+// reads use it with constant offsets (with the offset in a scalar register the compiler copies
+// the address into a vector register again in every forward's branch).
+template <typename R>
+__device__ __forceinline__ LmmPar<R> lmm_par(const LmmParRow<R>* rows, int off) {
+  asm volatile("" : "+v"(off));
+  return LmmPar<R>{rows + off};
+}
 
 template <typename R>
 __device__ __forceinline__ R lmm_recip(R x) {
@@ -252,7 +261,7 @@ sdesim_lmm(const LmmArgs a) {
     }
     R eps[F];
     lmm_noise<R>(rsrc, sb, n, eps);
-    lmm_step<R>(LmmPar<R>{par_rows, n * a.zero}, L, eps, n / a.steps_per_period + 1, dtu, sdtu, tau);
+    lmm_step<R>(lmm_par<R>(par_rows, n * a.zero), L, eps, n / a.steps_per_period + 1, dtu, sdtu, tau);
   }
   if (MODE == 2) {
     sde_tape_store<R, NF>(a.tape, N, S, voff, L);
@@ -282,7 +291,7 @@ sdesim_lmm(const LmmArgs a) {
         }
         R eps[F];
         lmm_noise<R>(rsrc, sb, n0 + k, eps);
-        lmm_step<R>(LmmPar<R>{par_rows, (n0 + k) * a.zero}, Xc, eps, (n0 + k) / a.steps_per_period + 1, dtu, sdtu,
+        lmm_step<R>(lmm_par<R>(par_rows, (n0 + k) * a.zero), Xc, eps, (n0 + k) / a.steps_per_period + 1, dtu, sdtu,
                     tau);
       }
       for (int k = len - 1; k >= 0; --k) {
@@ -290,7 +299,7 @@ sdesim_lmm(const LmmArgs a) {
 #pragma unroll
           for (int i = 0; i < NF; ++i) Xc[i] = tape[((long long)k * NF + i) * S + sid];
         }
-        lmm_step_adjoint<R>(LmmPar<R>{par_rows, (n0 + k) * a.zero}, Xc, Lp, Lb, (n0 + k) / a.steps_per_period + 1,
+        lmm_step_adjoint<R>(lmm_par<R>(par_rows, (n0 + k) * a.zero), Xc, Lp, Lb, (n0 + k) / a.steps_per_period + 1,
                             dt, tau);
 #pragma unroll
         for (int i = 0; i < NF; ++i) Lp[i] = Xc[i];
@@ -329,7 +338,7 @@ sdesim_lmm_reverse(const LmmArgs a) {
   const R V = lmm_payoff<R, true>(Lp, a.expiry_index, tau, K, Lb);
   sde_tape_load<R, NF>(a.tape, N - 1, S, voff, L);
   for (int n = N - 1; n >= 0; --n) {
-    const LmmPar<R> tab{par_rows, n * a.zero};
+    const LmmPar<R> tab = lmm_par<R>(par_rows, n * a.zero);
     const int eta = n / a.steps_per_period + 1;
     const __amdgpu_buffer_rsrc_t rs = sde_tape_rsrc<R, NF>(a.tape, n > 0 ? n - 1 : 0, S);
     R G[F] = {R(0), R(0), R(0)};
