@@ -441,3 +441,44 @@ def test_kernel_code_cache_on_disk(tmp_path, monkeypatch):
     again = open(path, "rb").read()  # recompiled and rewritten
     assert again != damaged and again[:8] == blob[:8] and len(again) > 24
     assert sorted(p for p in os.listdir(tmp_path)) == files  # no temporary files left behind
+
+
+def _random_universe(rng, jumps=False):
+    P = int(rng.integers(1, 3))
+    names = ["X", "Y"][:P]
+    eqs = []
+    for n in names:
+        terms = ["(0.1 * sin(%s)) * dt" % _random_coef(rng, names, 3)]
+        for k in range(int(rng.integers(1, 3))):
+            terms.append("(0.2 * cos(%s) * %s) * dW%d" % (_random_coef(rng, names, 3), n, k + 1))
+        if jumps:
+            terms.append("(0.05 * cos(%s)) * dN1(1 + abs(sin(%s)))" % (_random_coef(rng, names, 2),
+                                                                     _random_coef(rng, names, 2)))
+        eqs.append("d%s = %s" % (n, " + ".join(terms)))
+    return names, eqs
+
+
+def test_random_universes_fp32_and_jumps(oracle):
+    """More random equation sets: (a) FP32 kernels (reassociated within a step) against the oracle's
+    FP64 paths, every value within 1e-4 relative over 16 steps; (b) FP64 with a Poisson jump term
+    dN1(lambda(X)) in every equation (the jump count from the sequential inverse CDF of the same
+    uniform) against the oracle within 1e-10."""
+    rng = np.random.default_rng(33)
+    times = np.linspace(0.0, 1.0, 17)
+    for case in range(12):
+        jumps = case >= 6
+        names, eqs = _random_universe(rng, jumps=jumps)
+        scheme = str(rng.choice(["euler", "runge-kutta"]))
+        rng_method, scramble = [("pseudo", "shift"), ("sobol", "shift"), ("sobol", "xor")][int(rng.integers(0, 3))]
+        init = {n: float(rng.uniform(0.5, 1.5)) for n in names}
+        seed = int(rng.integers(0, 1000))
+        precision = "f64" if jumps else "f32"
+        print("case %d: %s %s %s/%s %s" % (case, precision, scheme, rng_method, scramble, eqs), flush=True)
+        _, f = _filtration(eqs, times, 300, init, rng_method, scheme, seed=seed, scramble=scramble,
+                           scramble_seed=seed + 1, precision=precision)
+        ref = oracle.simulate(eqs, times, 300, init, scheme=scheme, rng=rng_method, seed=seed, scramble=scramble,
+                              scramble_seed=seed + 1)
+        if jumps:
+            _close(f.transpose(2, 0, 1), ref, rtol=1e-10, atol=1e-12)
+        else:
+            _close(f.transpose(2, 0, 1), ref, rtol=1e-4, atol=1e-6)
