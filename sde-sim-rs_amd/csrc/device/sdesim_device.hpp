@@ -150,7 +150,8 @@ __device__ __forceinline__ float sde_inv_normal_f32_words(sde_u32 lo, sde_u32 hi
     const float den = ((0.001308f * t + 0.189269f) * t + 1.432788f) * t + 1.0f;
     x = t - num * __builtin_amdgcn_rcpf(den);
   }
-  return m ? x : -x;
+  // Negate when u < 1/2 (bit 31 of hi clear): flip the sign bit with the complement of hi's.
+  return __builtin_bit_cast(float, __builtin_bit_cast(sde_u32, x) ^ (~hi & 0x80000000u));
 }
 
 // Psi of the pseudo-random draw held as its two ChaCha words (lo, hi).
