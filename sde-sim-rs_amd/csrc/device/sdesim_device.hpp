@@ -205,7 +205,6 @@ __device__ __forceinline__ sde_u32 sde_q_index(sde_u32 yc) { return yc ^ (sde_u3
 // f32 inverse normal of (y + 0.5) 2^-32 from yc = ~y; the tail qi < 1233 (|z| > 5) takes the
 // f64 path.
 __device__ __forceinline__ float sde_inv_normal_f32_x(sde_u32 yc) {
-  const bool upper = (int)yc >= 0;
   const float q = sde_q_f32(yc);
   float x;
   if (sde_q_in_tail(q)) {
@@ -216,7 +215,8 @@ __device__ __forceinline__ float sde_inv_normal_f32_x(sde_u32 yc) {
     const float den = ((0.001308f * t + 0.189269f) * t + 1.432788f) * t + 1.0f;
     x = t - num * __builtin_amdgcn_rcpf(den);
   }
-  return upper ? x : -x;
+  // Negate when u < 1/2 (bit 31 of yc set): flip the sign bit with yc's.
+  return __builtin_bit_cast(float, __builtin_bit_cast(sde_u32, x) ^ (yc & 0x80000000u));
 }
 
 // Wiener increments sqrt(dt) * Psi(u) (src/proc/increment.rs:75-79) and the RK probe's
