@@ -382,3 +382,57 @@ def test_bench_refuses_more_than_one_gpu():
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in r.stderr
+
+
+def test_abi_argument_validation():
+    """Invalid arguments are refused with SDESIM_EVALUE and a message, before any device call
+    (so these run without a GPU): plan shapes and launch options, basket and forward-rate specs."""
+    u = sde_sim_rs.Universe(GBM)
+    times = np.ascontiguousarray([0.0, 0.5, 1.0])
+    x0 = np.ascontiguousarray([1.0])
+    plan = ctypes.c_void_p()
+
+    def plan_rc(S=8, **kw):
+        o = nat.default_options()
+        for k, v in kw.items():
+            setattr(o, k, v)
+        return nat.lib.sdesim_plan_create(u._h, nat.dptr(times), 3, nat.dptr(x0), S, ctypes.byref(o), None,
+                                          ctypes.byref(plan))
+
+    cases = [(dict(S=0), "scenarios must be a positive integer"),
+             (dict(block_size=96), "block_size must be a multiple of 64"),
+             (dict(block_size=2048), "block_size must be a multiple of 64"),
+             (dict(paths_per_lane=9), "paths_per_lane must be at most 8")]
+    for kw, msg in cases:
+        assert plan_rc(**kw) == nat.EVALUE, kw
+        assert msg in nat.last_error(), (kw, nat.last_error())
+
+    def basket_rc(**kw):
+        s = nat.BasketSpec()
+        s.assets, s.steps, s.paths, s.horizon, s.strike, s.width, s.precision = 8, 10, 1024, 1.0, 100.0, 0.5, nat.F32
+        for k, v in kw.items():
+            setattr(s, k, v)
+        h = ctypes.c_void_p()
+        return nat.lib.sdesim_basket_create(ctypes.byref(s), ctypes.byref(h))
+
+    for kw, msg in [(dict(assets=9), "assets must be between 1 and 8"), (dict(steps=0), "steps must be >= 1"),
+                    (dict(paths=0), "paths must be >= 1"), (dict(horizon=0.0), "horizon must be > 0"),
+                    (dict(width=-1.0), "width must be >= 0"), (dict(storage=3), "storage must be")]:
+        assert basket_rc(**kw) == nat.EVALUE, kw
+        assert msg in nat.last_error(), (kw, nat.last_error())
+
+    def lmm_rc(**kw):
+        s = nat.LmmSpec()
+        s.forwards, s.factors, s.steps, s.steps_per_period, s.expiry_index = 40, 3, 160, 40, 4
+        s.precision, s.paths, s.tenor, s.strike = nat.F32, 1024, 0.25, 0.03
+        for k, v in kw.items():
+            setattr(s, k, v)
+        h = ctypes.c_void_p()
+        return nat.lib.sdesim_lmm_create(ctypes.byref(s), ctypes.byref(h))
+
+    for kw, msg in [(dict(forwards=20), "compiled for 40 forwards"), (dict(expiry_index=40), "expiry_index out of range"),
+                    (dict(steps=150), "steps must equal steps_per_period * expiry_index"),
+                    (dict(paths=(1 << 22) + 1), "paths must be in [1, 2^22]"), (dict(tenor=0.0), "tenor must be > 0"),
+                    (dict(storage=-1), "storage must be")]:
+        assert lmm_rc(**kw) == nat.EVALUE, kw
+        assert msg in nat.last_error(), (kw, nat.last_error())
