@@ -482,3 +482,15 @@ def test_random_universes_fp32_and_jumps(oracle):
             _close(f.transpose(2, 0, 1), ref, rtol=1e-10, atol=1e-12)
         else:
             _close(f.transpose(2, 0, 1), ref, rtol=1e-4, atol=1e-6)
+
+
+def test_long_time_grid(oracle):
+    """A long grid: 10,000 steps of GBM Runge-Kutta with Sobol-XOR uniforms (10,000 dimensions of
+    the direction-number table), 1000 paths, FP64 at the parity bar."""
+    times = np.linspace(0.0, 10.0, 10001)
+    S = 1000
+    print("case: long grid gbm rk sobol-xor S=%d N=10000" % S, flush=True)
+    _, f = _filtration(GBM, times, S, {"X": 100.0}, "sobol", "runge-kutta", scramble="xor", scramble_seed=4)
+    ref = oracle.simulate(GBM, times, S, {"X": 100.0}, scheme="runge-kutta", rng="sobol", scramble="xor",
+                          scramble_seed=4)
+    _close(f.transpose(2, 0, 1), ref)
