@@ -45,25 +45,31 @@ __global__ void __launch_bounds__(256) sdesim_finalize(const double* __restrict_
 // Long layout: [rows][S] (rows = T * P) to [S][rows] f64. A block moves a tile of 256 rows x W
 // scenarios through LDS (W = 32 for f32 input, 16 for f64: 128 bytes of one row per 8 lanes):
 //  * reads: each lane loads 16 bytes (4 f32 or 2 f64 consecutive scenarios), a whole 128-byte
-//    line of a row per 8 lanes;
+//    line of a row per 8 lanes, with the non-temporal policy (each input byte is read once);
 //  * writes: for each scenario, the tile's 256 rows as f64 (2 KB contiguous), each lane storing
 //    two adjacent rows in one 16-byte non-temporal store (the output is written once and not
 //    read back by this kernel);
 //  * the LDS row pitch is W + 1 elements.
+// Tiles are numbered with the row tile fastest (tile b covers row tile b % ny of scenario group
+// b / ny), so the blocks resident together fill whole output rows of a few scenario groups instead
+// of 2 KB pieces of every scenario. One block per tile: blocks that loop over several tiles were
+// 7 % slower.
 // Among the layouts measured on 2050 x 2^20 values (64 x 128 up to 512 x 16 tiles, 8- or 16-byte
-// stores, default or non-temporal policy) this one moved the most bytes per second. Tiles that are
-// not whole (the last rows or scenarios), or data whose rows are not 16-byte aligned (S not a
-// multiple of the vector width, an odd row count), take element-wise guarded accesses.
+// stores, default or non-temporal policy, scenario- or row-tile-fastest order, one tile per block
+// or a block walking all row tiles of its scenarios) this one moved the most bytes per second.
+// Tiles that are not whole (the last rows or scenarios), or data whose rows are not 16-byte aligned
+// (S not a multiple of the vector width, an odd row count), take element-wise guarded accesses.
 template <typename T>
-__global__ void __launch_bounds__(256) sdesim_to_long(const T* __restrict__ in, long long S, int rows,
+__global__ void __launch_bounds__(256) sdesim_to_long(const T* __restrict__ in, long long S, int rows, int ny,
                                                       double* __restrict__ out) {
   constexpr int VEC = 16 / (int)sizeof(T);
   constexpr int W = 8 * VEC;
   constexpr int RT = 256;
   __shared__ T tile[RT][W + 1];
-  const long long s0 = (long long)blockIdx.x * W;
-  const int r0 = blockIdx.y * RT;
   const int t = threadIdx.x;
+  const long long b = blockIdx.x;
+  const long long s0 = (b / ny) * W;
+  const int r0 = (int)(b % ny) * RT;
   const bool whole = r0 + RT <= rows && s0 + W <= S && S % VEC == 0;
 #pragma unroll
   for (int pass = 0; pass < RT / 32; ++pass) {
@@ -74,7 +80,7 @@ __global__ void __launch_bounds__(256) sdesim_to_long(const T* __restrict__ in, 
     T v[VEC];
     if (whole) {
       typedef T TV __attribute__((ext_vector_type(VEC)));
-      const TV x = *(const TV*)(in + (long long)row * S + s);
+      const TV x = __builtin_nontemporal_load((const TV*)(in + (long long)row * S + s));
 #pragma unroll
       for (int e = 0; e < VEC; ++e) v[e] = x[e];
     } else {
@@ -117,11 +123,14 @@ hipError_t sdesim_launch_finalize(const double* partials, int nblocks, int kq, l
 hipError_t sdesim_launch_to_long(const void* filtration, int f32, long long S, int rows, double* out,
                                  hipStream_t stream) {
   const long long W = f32 ? 32 : 16;  // scenarios per tile (256 rows per tile)
-  dim3 grid((unsigned)((S + W - 1) / W), (unsigned)((rows + 255) / 256));
-  if (grid.y > 65535u) return hipErrorInvalidValue;
+  const int ny = (rows + 255) / 256;
+  const long long ntiles = ((S + W - 1) / W) * ny;
+  // A dispatch holds fewer than 2^32 work-items. 2^24 tiles of 256 x W would be 550 GB of input.
+  if (ntiles >= (1LL << 24)) return hipErrorInvalidValue;
+  const dim3 grid((unsigned)ntiles);
   if (f32)
-    hipLaunchKernelGGL(sdesim_to_long<float>, grid, dim3(256), 0, stream, (const float*)filtration, S, rows, out);
+    hipLaunchKernelGGL(sdesim_to_long<float>, grid, dim3(256), 0, stream, (const float*)filtration, S, rows, ny, out);
   else
-    hipLaunchKernelGGL(sdesim_to_long<double>, grid, dim3(256), 0, stream, (const double*)filtration, S, rows, out);
+    hipLaunchKernelGGL(sdesim_to_long<double>, grid, dim3(256), 0, stream, (const double*)filtration, S, rows, ny, out);
   return hipGetLastError();
 }
