@@ -11,6 +11,7 @@
 
 #include <unistd.h>
 
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -666,7 +667,14 @@ int sdesim_filtration_to_long(const void* d_filtration, int32_t precision, int64
                               double* d_long, void* stream) {
   return guarded([&] {
     if (S <= 0 || T <= 0 || P < 0) fail(SDESIM_EVALUE, "bad shape");
+    if (precision != SDESIM_F32 && precision != SDESIM_F64) fail(SDESIM_EVALUE, "precision must be f32 or f64");
+    if ((int64_t)T * P > INT32_MAX) fail(SDESIM_EVALUE, "n_times * n_processes exceeds 2^31 - 1");
     if (P == 0) return;
+    // The transpose runs one block per tile of 256 rows x 32 (f32) or 16 (f64) scenarios.
+    const int64_t W = precision == SDESIM_F32 ? 32 : 16;
+    if (((S + W - 1) / W) * (((int64_t)T * P + 255) / 256) >= (int64_t(1) << 24))
+      fail(SDESIM_EVALUE, "filtration too large for one long-layout transpose (2^24 tiles of 256 rows)");
+    if (!d_filtration || !d_long) fail(SDESIM_EVALUE, "null argument");
     hip_check(sdesim_launch_to_long(d_filtration, precision == SDESIM_F32 ? 1 : 0, S, T * P, d_long,
                                     (hipStream_t)stream),
               "to_long");

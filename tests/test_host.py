@@ -436,3 +436,14 @@ def test_abi_argument_validation():
                     (dict(storage=-1), "storage must be")]:
         assert lmm_rc(**kw) == nat.EVALUE, kw
         assert msg in nat.last_error(), (kw, nat.last_error())
+
+    # The long-layout transpose: shapes are checked before any device call (null buffers here).
+    to_long = nat.lib.sdesim_filtration_to_long
+    for args, msg in [((1, 0, 4, 2), "bad shape"), ((7, 8, 4, 2), "precision must be f32 or f64"),
+                      ((1, 8, 1 << 16, 1 << 16), "exceeds 2^31 - 1"),
+                      ((1, 1 << 40, 1025, 2), "too large for one long-layout transpose"),
+                      ((0, 1 << 39, 1025, 2), "too large for one long-layout transpose"),
+                      ((1, 8, 4, 2), "null argument")]:
+        precision, S, T, P = args
+        assert to_long(None, precision, S, T, P, None, None) == nat.EVALUE, args
+        assert msg in nat.last_error(), (args, nat.last_error())
