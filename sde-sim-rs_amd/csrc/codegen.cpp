@@ -16,6 +16,17 @@
 // the reference computes twice from the same cache and time (e.g. RK stage-2 coefficients at
 // t_{n+1} and the next step's stage-1 coefficients at the same time on the same probe cache)
 // are carried in registers across steps instead of being recomputed.
+//
+// FP32 kernels also move constant factors from coefficients to increments, where they cost
+// less (the values are those of the same expressions at the same points; only the association
+// of the products changes):
+//   * a dt term c * r(t, C) becomes r * (c * dt), and c * dt is loop-invariant on a uniform grid;
+//   * Wiener terms of one process whose coefficients are constant multiples c_k * r of one
+//     residual r become r * g with g = sum_k c_k * dW_k formed once per step, and their RK probe
+//     terms r * ((sum_k c_k) * sk * sqrt(dt));
+//   * an increment whose every use is such a group member with one constant c > 0 is formed as
+//     sqrt(dt) * c * Psi(u) directly (the multiplier takes its sign from u, so c must be > 0).
+#include <cstdio>
 #include <map>
 #include <set>
 #include <sstream>
@@ -37,6 +48,15 @@ class StepEmitter {
  public:
   StepEmitter(const Universe& u, const CodegenOptions& o) : u_(u), o_(o) {
     R_ = o.f32 ? "float" : "double";
+    if (o.f32) plan_terms();
+  }
+
+  // Residual expressions made by the FP32 term plans, by canonical text (values carried across
+  // steps may be residuals).
+  void add_residuals(std::map<std::string, const Expr*>& all) const {
+    for (const auto& terms : plan_)
+      for (const TermPlan& tp : terms)
+        if (tp.r) all[canonical(*tp.r)] = tp.r;
   }
 
   // Emit one step given the cache state at entry. `carried` lists values available in K[i]
@@ -45,6 +65,7 @@ class StepEmitter {
     out_.str("");
     memo_.clear();
     psi_.clear();
+    gvar_.clear();
     at_ = entry;
     gen_ = 0;
     tmp_ = 0;
@@ -122,6 +143,151 @@ class StepEmitter {
   int end_gen_ = 0;
   std::vector<std::string> end_values_;
 
+  // FP32 term plans, per process and term: kind 0 = the coefficient as written, 1 = a dt term
+  // r * (c * dt), 2 = a member of Wiener group `group` (r * g).
+  struct TermPlan {
+    int kind = 0;
+    double c = 1.0;
+    const Expr* r = nullptr;
+    int group = -1;
+  };
+  struct Group {
+    const Expr* r;
+    std::vector<std::pair<double, int>> members;  // (c_k, increment index)
+    double csum = 0.0;
+  };
+  struct Split {
+    double c;
+    const Expr* r;  // nullptr: the expression is the constant c
+  };
+  std::vector<std::vector<TermPlan>> plan_;
+  std::vector<Group> groups_;
+  std::vector<ExprPtr> store_;
+  std::map<int, std::string> gvar_;
+  std::map<int, double> prescale_;  // increment index -> the constant folded into its multiplier
+
+  static ExprPtr clone(const Expr& e) {
+    ExprPtr c = std::make_unique<Expr>();
+    c->op = e.op;
+    c->fn = e.fn;
+    c->value = e.value;
+    c->name = e.name;
+    for (const ExprPtr& a : e.args) c->args.push_back(clone(*a));
+    return c;
+  }
+
+  // e = c * r with c constant: peels constant factors off products and negations.
+  Split split(const Expr& e) {
+    if (e.op == Op::Const) return {e.value, nullptr};
+    if (e.op == Op::Neg) {
+      Split a = split(*e.args[0]);
+      return {-a.c, a.r};
+    }
+    if (e.op == Op::Mul) {
+      Split a = split(*e.args[0]), b = split(*e.args[1]);
+      if (!a.r) return {a.c * b.c, b.r};
+      if (!b.r) return {a.c * b.c, a.r};
+      if (a.c == 1.0 && b.c == 1.0) return {1.0, &e};
+      ExprPtr m = std::make_unique<Expr>();
+      m->op = Op::Mul;
+      m->args.push_back(clone(*a.r));
+      m->args.push_back(clone(*b.r));
+      store_.push_back(std::move(m));
+      return {a.c * b.c, store_.back().get()};
+    }
+    return {1.0, &e};
+  }
+
+  void plan_terms() {
+    plan_.assign(u_.P(), {});
+    for (int p = 0; p < u_.P(); ++p) {
+      const ProcessDef& pd = u_.processes[p];
+      if (!pd.levy) continue;
+      std::vector<Split> sp;
+      std::map<std::string, std::vector<size_t>> by_r;
+      for (size_t k = 0; k < pd.terms.size(); ++k) {
+        const Term& t = pd.terms[k];
+        sp.push_back(split(*t.coef));
+        plan_[p].push_back(TermPlan{});
+        if (!sp[k].r) continue;
+        if (t.kind == IncKind::Time && sp[k].c != 1.0) plan_[p][k] = TermPlan{1, sp[k].c, sp[k].r, -1};
+        if (t.kind == IncKind::Wiener) by_r[canonical(*sp[k].r)].push_back(k);
+      }
+      for (auto& kv : by_r) {
+        if (kv.second.size() < 2) continue;
+        Group g{sp[kv.second[0]].r, {}, 0.0};
+        for (size_t k : kv.second) {
+          g.members.push_back({sp[k].c, pd.terms[k].index});
+          g.csum += sp[k].c;
+          plan_[p][k] = TermPlan{2, sp[k].c, g.r, (int)groups_.size()};
+        }
+        groups_.push_back(g);
+      }
+    }
+    // Increments used only as group members with one constant c > 0.
+    std::map<int, std::set<double>> consts;
+    std::set<int> plain;
+    for (int p = 0; p < u_.P(); ++p) {
+      const ProcessDef& pd = u_.processes[p];
+      if (!pd.levy) continue;
+      for (size_t k = 0; k < pd.terms.size(); ++k) {
+        if (pd.terms[k].kind != IncKind::Wiener) continue;
+        if (plan_[p][k].kind == 2) consts[pd.terms[k].index].insert(plan_[p][k].c);
+        else plain.insert(pd.terms[k].index);
+      }
+    }
+    for (auto& kv : consts)
+      if (!plain.count(kv.first) && kv.second.size() == 1 && *kv.second.begin() > 0.0)
+        prescale_[kv.first] = *kv.second.begin();
+  }
+
+  static std::string flit(double v) {
+    char b[64];
+    std::snprintf(b, sizeof b, "((float)%a)", v);
+    return b;
+  }
+
+  // g = sum_k c_k * dW_k of a Wiener group, once per step.
+  std::string group_var(int gid) {
+    auto it = gvar_.find(gid);
+    if (it != gvar_.end()) return it->second;
+    const Group& g = groups_[gid];
+    std::string expr;
+    for (const auto& m : g.members)
+      expr += (expr.empty() ? "" : " + ") + (prescale_.count(m.second) ? dw(m.second) : flit(m.first) + " * " + dw(m.second));
+    std::string name = "g" + std::to_string(gid);
+    line("const " + R_ + " " + name + " = " + expr + ";");
+    gvar_[gid] = name;
+    return name;
+  }
+
+  // Emits sum = (the products of process p's terms at t_n (n1 = false) or t_{n+1}, increments
+  // d[k]), each line right after the values it reads; `first` is the text that starts the sum
+  // ("R sum = " or "sum = "). Returns the number of products.
+  int sum_terms(int p, bool n1, const std::vector<std::string>& d, const std::string& sum, const std::string& first) {
+    const ProcessDef& pd = u_.processes[p];
+    std::set<int> done;
+    int n = 0;
+    for (size_t k = 0; k < pd.terms.size(); ++k) {
+      const TermPlan tp = plan_.empty() ? TermPlan{} : plan_[p][k];
+      std::string term;
+      if (tp.kind == 0) {
+        term = eval(*pd.terms[k].coef, n1) + " * " + d[k];
+      } else if (tp.kind == 1) {
+        term = eval(*tp.r, n1) + " * (" + flit(tp.c) + " * dt)";
+      } else {
+        if (done.count(tp.group)) continue;
+        done.insert(tp.group);
+        std::string r = eval(*tp.r, n1);
+        term = r + " * " + group_var(tp.group);
+      }
+      if (n == 0) line(first + term + ";");
+      else line(sum + " = " + sum + " + " + term + ";");
+      ++n;
+    }
+    return n;
+  }
+
   MemoKey key(const std::string& canon, bool n1) const { return MemoKey{canon, n1, gen_}; }
   void line(const std::string& s) { out_ << "    " << s << "\n"; }
   std::string y(int p) const { return "y" + std::to_string(p); }
@@ -151,7 +317,9 @@ class StepEmitter {
   std::string dw(int idx) {
     std::string w = "w" + std::to_string(idx);
     if (!psi_.count(idx)) {
-      line("const " + R_ + " " + w + " = sde_dw(u[" + std::to_string(idx) + "], sdt);");
+      auto pc = prescale_.find(idx);
+      const std::string m = pc == prescale_.end() ? std::string("sdt") : "(" + flit(pc->second) + " * sdt)";
+      line("const " + R_ + " " + w + " = sde_dw(u[" + std::to_string(idx) + "], " + m + ");");
       psi_.insert(idx);
     }
     return w;
@@ -192,15 +360,9 @@ class StepEmitter {
       std::string acc = "a" + std::to_string(p);
       if (o_.f32) {
         // f32: the terms are summed first and the sum is added to the state strictly.
-        bool first = true;
-        for (const Term& t : pd.terms) {
-          std::string c = eval(*t.coef, false);
-          std::string x = sample(t);
-          if (first) line(R_ + " " + acc + " = " + c + " * " + x + ";");
-          else line(acc + " = " + acc + " + " + c + " * " + x + ";");
-          first = false;
-        }
-        if (first) line(y(p) + " = X[" + std::to_string(p) + "];");
+        std::vector<std::string> d;
+        for (const Term& t : pd.terms) d.push_back(sample(t));
+        if (sum_terms(p, false, d, acc, R_ + " " + acc + " = ") == 0) line(y(p) + " = X[" + std::to_string(p) + "];");
         else line(y(p) + " = sde_add_strict(X[" + std::to_string(p) + "], " + acc + ");");
         continue;
       }
@@ -235,11 +397,7 @@ class StepEmitter {
       if (!pd.levy) continue;
       std::string k1 = "k1_" + std::to_string(p);
       line(R_ + " " + k1 + " = " + R_ + "(0);");
-      for (size_t k = 0; k < pd.terms.size(); ++k) {
-        std::string c = eval(*pd.terms[k].coef, false);
-        if (k == 0) line(k1 + " = " + c + " * " + d[p][k] + ";");
-        else line(k1 + " = " + k1 + " + " + c + " * " + d[p][k] + ";");
-      }
+      sum_terms(p, false, d[p], k1, k1 + " = ");
     }
     for (int p = 0; p < P; ++p) {
       const ProcessDef& pd = u_.processes[p];
@@ -247,11 +405,21 @@ class StepEmitter {
       std::string pe = "pe" + std::to_string(p);
       line(R_ + " " + pe + " = " + R_ + "(0);");
       bool first = true;
-      for (const Term& t : pd.terms) {
+      std::set<int> done;
+      for (size_t k = 0; k < pd.terms.size(); ++k) {
+        const Term& t = pd.terms[k];
         if (t.kind != IncKind::Wiener) continue;
-        std::string c = eval(*t.coef, false);
-        if (first) line(pe + " = " + c + " * sks;");
-        else line(pe + " = " + pe + " + " + c + " * sks;");
+        const TermPlan tp = plan_.empty() ? TermPlan{} : plan_[p][k];
+        std::string term;
+        if (tp.kind == 2) {
+          if (done.count(tp.group)) continue;
+          done.insert(tp.group);
+          term = eval(*tp.r, false) + " * (" + flit(groups_[tp.group].csum) + " * sks)";
+        } else {
+          term = eval(*t.coef, false) + " * sks";
+        }
+        if (first) line(pe + " = " + term + ";");
+        else line(pe + " = " + pe + " + " + term + ";");
         first = false;
       }
       line(y(p) + " = X[" + std::to_string(p) + "] + k1_" + std::to_string(p) + " + " + pe + ";");
@@ -261,11 +429,7 @@ class StepEmitter {
       if (!pd.levy) continue;
       std::string k2 = "k2_" + std::to_string(p);
       line(R_ + " " + k2 + " = " + R_ + "(0);");
-      for (size_t k = 0; k < pd.terms.size(); ++k) {
-        std::string c = eval(*pd.terms[k].coef, true);
-        if (k == 0) line(k2 + " = " + c + " * " + d[p][k] + ";");
-        else line(k2 + " = " + k2 + " + " + c + " * " + d[p][k] + ";");
-      }
+      sum_terms(p, true, d[p], k2, k2 + " = ");
     }
     for (int p : u_.levy) {
       const std::string inc = R_ + "(0.5) * (k1_" + std::to_string(p) + " + k2_" + std::to_string(p) + ")";
@@ -302,6 +466,7 @@ std::string generate_kernel_source(const Universe& u, const CodegenOptions& o) {
         if (t.lambda) all[canonical(*t.lambda)] = t.lambda.get();
       }
     }
+    em.add_residuals(all);
     for (const std::string& c : em.end_values()) {
       auto it = all.find(c);
       if (it != all.end()) carried.push_back(Carried{c, it->second});

@@ -215,11 +215,44 @@ def test_generated_kernels_compile(eqs, scheme, precision, rng):
 def test_codegen_reference_rk_carries_probe_coefficients():
     # RK with reference compat: the stage-1 coefficients of step n are the stage-2 values of step
     # n-1 (same expressions on the same probe cache at the same time), carried in registers.
-    src = generate(HESTON, 33, scheme=1, precision=1, rng=1, scramble=1)
-    body = src[src.index("void step"):]
-    assert "K[" in body and body.count("sde_sqrt(") == 3
-    src_tb = generate(HESTON, 33, scheme=1, precision=1, rng=1, scramble=1, compat=1)
-    assert src_tb[src_tb.index("void step"):].count("sde_sqrt(") == 6
+    # FP64: three coefficients hold a square root (sqrt(v) S, -0.35 sqrt(v), 0.357 sqrt(v)); with
+    # the textbook cache both stages evaluate them. FP32 factors sqrt(v) out of the variance's two
+    # Wiener coefficients, so two expressions hold one.
+    for precision, once in ((0, 3), (1, 2)):
+        src = generate(HESTON, 33, scheme=1, precision=precision, rng=1, scramble=1)
+        body = src[src.index("void step"):]
+        assert "K[" in body and body.count("sde_sqrt(") == once
+        src_tb = generate(HESTON, 33, scheme=1, precision=precision, rng=1, scramble=1, compat=1)
+        assert src_tb[src_tb.index("void step"):].count("sde_sqrt(") == 2 * once
+
+
+def test_codegen_fp32_factors_constants_out_of_coefficients():
+    # FP32: c * r dt terms become r * (c * dt); the variance's Wiener terms -0.35 sqrt(v) dW1 and
+    # 0.357 sqrt(v) dW2 become sqrt(v) * g with g = -0.35 dW1 + dW2', where dW2' = 0.357 sqrt(dt)
+    # Psi(u_1) (dW2 has no other use and 0.357 > 0); the RK probe term is
+    # sqrt(v) * ((-0.35 + 0.357) * sk sqrt(dt)). FP64 keeps every coefficient as written.
+    def hx(v):  # C's %a form of a double
+        mant, exp = float.hex(v).split("p")
+        return mant.rstrip("0").rstrip(".") + "p" + exp
+
+    for scheme in (0, 1):
+        src = generate(HESTON, 33, scheme=scheme, precision=1, rng=1, scramble=1)
+        step = src[src.index("void step"):src.index("payoff")]
+        c1, c2 = -0.7 * 0.5, 0.7141428428542850 * 0.5
+        assert "w1 = sde_dw(u[1], (((float)%s) * sdt));" % hx(c2) in step
+        assert "g0 = ((float)%s) * w0 + w1;" % hx(c1) in step
+        assert "* (((float)%s) * dt)" % hx(0.02) in step  # 0.02 * S
+        assert "* (((float)%s) * dt)" % hx(2.0) in step  # 2 * (0.04 - max(v, 0))
+        assert "w0 = sde_dw(u[0], sdt);" in step  # dW1 is also used by dS, unscaled
+        if scheme == 1:
+            assert "* (((float)%s) * sks)" % hx(c1 + c2) in step
+        f64 = generate(HESTON, 33, scheme=scheme, precision=0, rng=1, scramble=1)
+        assert "g0" not in f64 and "* dt)" not in f64 and "sde_dw(u[1], sdt)" in f64
+    # A constant < 0 is not folded into the multiplier (the XOR form takes the sign from u).
+    neg = ["dX = (0.1 * X) * dt + (0.2 * X) * dW1 + (-0.3 * X) * dW2"]
+    src = generate(neg, 9, scheme=0, precision=1, rng=1, scramble=1)
+    assert "w0 = sde_dw(u[0], (((float)%s) * sdt));" % hx(0.2) in src and "w1 = sde_dw(u[1], sdt);" in src
+    assert "g0 = w0 + ((float)%s) * w1;" % hx(-0.3) in src
 
 
 def test_codegen_fp32_reassociation_is_confined_to_the_step():
