@@ -111,7 +111,9 @@ def _profiled_traffic(paths, nsteps):
     return None
 
 
-def cpu_baseline(paths, steps_n, threads):
+def cpu_baseline(paths, steps_n, threads, runs=3):
+    """The oracle (the reference simulator's CPU restatement) on `paths` paths of the headline
+    workload, `threads` threads; the median of `runs` timed runs after a small warm-up."""
     import numpy as np
 
     from oracle import oracle as O
@@ -119,11 +121,14 @@ def cpu_baseline(paths, steps_n, threads):
     times = np.linspace(0.0, 1.0, steps_n + 1)
     O.simulate(HESTON, times, 64, INIT, scheme="runge-kutta", rng="sobol", scramble="xor", scramble_seed=7,
                threads=threads)
-    t0 = time.perf_counter()
-    O.simulate(HESTON, times, paths, INIT, scheme="runge-kutta", rng="sobol", scramble="xor", scramble_seed=7,
-               threads=threads)
-    dt = time.perf_counter() - t0
-    return paths * steps_n / dt, dt
+    dts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        O.simulate(HESTON, times, paths, INIT, scheme="runge-kutta", rng="sobol", scramble="xor", scramble_seed=7,
+                   threads=threads)
+        dts.append(time.perf_counter() - t0)
+    dt = sorted(dts)[runs // 2]
+    return paths * steps_n / dt, dts
 
 
 def main():
@@ -135,7 +140,7 @@ def main():
     ap.add_argument("--nsteps", type=int, default=1024)
     ap.add_argument("--ppl", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
-    ap.add_argument("--cpu-paths", type=int, default=1 << 18)
+    ap.add_argument("--cpu-paths", type=int, default=1 << 16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib", action="store_true", help="run a 1 GiB device copy first (HBM counter calibration)")
     ap.add_argument("--only", default="", help="comma list of legs to run: f32,long,f64,term,batch,adjoint,lmm")
@@ -346,10 +351,11 @@ def main():
         line["forward_rate_model"] = lmm
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
-        v, dt = cpu_baseline(args.cpu_paths, N, threads)
+        v, dts = cpu_baseline(args.cpu_paths, N, threads)
         line["cpu_baseline"] = {"value": v, "unit": "path-steps/s", "cores": threads, "kind": "port",
-                                "sample": "same workload, %d paths x %d steps, oracle restatement of the "
-                                          "reference simulator (f64, full filtration), %.1f s" % (args.cpu_paths, N, dt)}
+                                "sample": "same workload, %d paths x %d steps, oracle restatement of the reference "
+                                          "simulator (f64, full filtration); median of 3 runs (%s s)"
+                                          % (args.cpu_paths, N, ", ".join("%.1f" % d for d in dts))}
     print(json.dumps(line))
 
 
