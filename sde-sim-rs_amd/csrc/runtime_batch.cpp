@@ -10,6 +10,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -143,12 +144,16 @@ struct sdesim_batch {
 
   // (Re)build the work queue for a paths-per-lane value and size the persistent grid.
   void configure(int new_ppl, int new_block) {
+    const long long per_item = 64LL * new_ppl;
+    long long total_chunks = 0;
+    for (int i = 0; i < n; ++i) total_chunks += (host[i].paths + per_item - 1) / per_item;
+    if (total_chunks > (long long)INT32_MAX)
+      throw BatchFailure{SDESIM_EVALUE, "too many work items (paths / (64 * paths_per_lane) summed over the batch)"};
     // Buffers below may be in use by queued launches of the previous configuration.
     if (d_items) bcheck(hipDeviceSynchronize(), "hipDeviceSynchronize");
     drop_graph();
     ppl = new_ppl;
     block = new_block;
-    const long long per_item = 64LL * ppl;
     std::vector<int> slot0(n + 1, 0);
     std::vector<SdeItem> items;
     for (int i = 0; i < n; ++i) {
@@ -223,6 +228,10 @@ int sdesim_batch_create(const sdesim_instance* inst, int32_t n, int32_t precisio
       mix((unsigned long long)a.kind); mix((unsigned long long)a.steps); mix((unsigned long long)a.paths);
     }
     b->total_paths = b->term_off[n];
+    long long items1 = 0;  // work items at one path per lane (the most)
+    for (int i = 0; i < n; ++i) items1 += (inst[i].paths + 63) / 64;
+    if (items1 > (long long)INT32_MAX)
+      throw BatchFailure{SDESIM_EVALUE, "too many work items (paths / (64 * paths_per_lane) summed over the batch)"};
     char kb[64];
     std::snprintf(kb, sizeof kb, "%s-%d-%016llx", b->f32 ? "f32" : "f64", n, sig);
     b->key = kb;
@@ -264,9 +273,16 @@ int sdesim_batch_autotune(sdesim_batch* b, void* stream, int32_t* ppl_out, int32
       best_p = cp;
       best_b = cb;
     } else {
-      hipEvent_t e0, e1;
-      bcheck(hipEventCreate(&e0), "hipEventCreate");
-      bcheck(hipEventCreate(&e1), "hipEventCreate");
+      struct Events {
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        ~Events() {
+          if (e0) (void)hipEventDestroy(e0);
+          if (e1) (void)hipEventDestroy(e1);
+        }
+      } ev;
+      bcheck(hipEventCreate(&ev.e0), "hipEventCreate");
+      bcheck(hipEventCreate(&ev.e1), "hipEventCreate");
+      hipEvent_t e0 = ev.e0, e1 = ev.e1;
       for (int p : {1, 2, 4}) {
         for (int blk : {64, 128, 256}) {
           b->configure(p, blk);
@@ -284,8 +300,6 @@ int sdesim_batch_autotune(sdesim_batch* b, void* stream, int32_t* ppl_out, int32
           if (best < 0 || ms_best < best) { best = ms_best; best_p = p; best_b = blk; }
         }
       }
-      (void)hipEventDestroy(e0);
-      (void)hipEventDestroy(e1);
       tune_store(b->key, best_p, best_b, best);
     }
     b->configure(best_p, best_b);

@@ -470,6 +470,21 @@ def test_abi_argument_validation():
         assert lmm_rc(**kw) == nat.EVALUE, kw
         assert msg in nat.last_error(), (kw, nat.last_error())
 
+    # Batches: instance parameters and the work-item count (paths / 64 summed over the batch must
+    # fit 32 bits).
+    def batch_rc(insts):
+        arr = (nat.Instance * len(insts))(*insts)
+        h = ctypes.c_void_p()
+        return nat.lib.sdesim_batch_create(arr, len(insts), 1, 0, ctypes.byref(h))
+
+    ok = dict(kind=0, steps=10, paths=1000, horizon=1.0)
+    for bad, msg in [(dict(kind=3), "unknown model"), (dict(steps=0), "steps must be >= 1"),
+                     (dict(paths=0), "paths must be >= 1"), (dict(horizon=0.0), "horizon must be > 0")]:
+        assert batch_rc([nat.Instance(**dict(ok, **bad))]) == nat.EVALUE, bad
+        assert msg in nat.last_error(), (bad, nat.last_error())
+    huge = [nat.Instance(**dict(ok, paths=1 << 36)) for _ in range(3)]  # 3 * 2^30 items at 64 paths each
+    assert batch_rc(huge) == nat.EVALUE and "too many work items" in nat.last_error()
+
     # The long-layout transpose: shapes are checked before any device call (null buffers here).
     to_long = nat.lib.sdesim_filtration_to_long
     for args, msg in [((1, 0, 4, 2), "bad shape"), ((7, 8, 4, 2), "precision must be f32 or f64"),
