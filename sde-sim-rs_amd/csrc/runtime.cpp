@@ -690,6 +690,10 @@ int sdesim_simulate_long(const sdesim_universe* u, const double* times, int32_t 
   int rc = guarded([&] {
     if (!u) fail(SDESIM_EVALUE, "null universe");
     if (S <= 0) fail(SDESIM_EVALUE, "scenarios must be a positive integer");
+    if (!h_values) fail(SDESIM_EVALUE, "null output buffer");
+    if (n_init < 0 || (n_init > 0 && (!init_names || !init_values))) fail(SDESIM_EVALUE, "bad initial values");
+    for (int32_t i = 0; i < n_init; ++i)
+      if (!init_names[i]) fail(SDESIM_EVALUE, "null initial value name");
     const int P = u->u.P();
     std::vector<double> x0(P, 0.0);
     for (int32_t i = 0; i < n_init; ++i) {

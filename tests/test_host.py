@@ -485,6 +485,19 @@ def test_abi_argument_validation():
     huge = [nat.Instance(**dict(ok, paths=1 << 36)) for _ in range(3)]  # 3 * 2^30 items at 64 paths each
     assert batch_rc(huge) == nat.EVALUE and "too many work items" in nat.last_error()
 
+    # The one-call long simulation checks its pointers before planning.
+    vals = np.zeros(3 * 8)
+    names = (ctypes.c_char_p * 1)(b"X")
+    inits = (ctypes.c_double * 1)(1.0)
+    sim = nat.lib.sdesim_simulate_long
+    assert sim(u._h, nat.dptr(times), 3, names, inits, 1, 8, None, None, None) == nat.EVALUE
+    assert "null output buffer" in nat.last_error()
+    assert sim(u._h, nat.dptr(times), 3, None, None, 1, 8, None, nat.dptr(vals), None) == nat.EVALUE
+    assert "bad initial values" in nat.last_error()
+    assert sim(u._h, nat.dptr(times), 3, (ctypes.c_char_p * 1)(None), inits, 1, 8, None, nat.dptr(vals),
+               None) == nat.EVALUE
+    assert "null initial value name" in nat.last_error()
+
     # The long-layout transpose: shapes are checked before any device call (null buffers here).
     to_long = nat.lib.sdesim_filtration_to_long
     for args, msg in [((1, 0, 4, 2), "bad shape"), ((7, 8, 4, 2), "precision must be f32 or f64"),
