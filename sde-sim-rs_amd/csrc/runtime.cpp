@@ -20,6 +20,7 @@
 #include <fstream>
 #include <map>
 #include <sstream>
+#include <stdexcept>
 #include <memory>
 #include <mutex>
 #include <set>
@@ -282,6 +283,7 @@ SobolTables sobol_tables(int dims, int nslices) {
 
 // Sliced XOR tables [dims][64 + 256 (nslices - 1)] of the direction numbers.
 void build_sobol_tables(int want, int nslices, std::vector<unsigned int>& host) {
+  if (want > sdesim_jk_dims) throw std::out_of_range("Sobol dimension exceeds the direction-number table");
   const int stride = 64 + 256 * (nslices - 1);
   host.assign((size_t)want * stride, 0u);
   for (int d = 0; d < want; ++d) {

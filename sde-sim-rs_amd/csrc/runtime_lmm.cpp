@@ -16,6 +16,7 @@ hipError_t sdesim_launch_sum_partials(const double* partials, int nblocks, int n
 int sdesim_set_error(int code, const std::string& msg);
 void sdesim_host_sobol_tables(int dims, int nslices, std::vector<unsigned int>& out);
 void sdesim_chacha8_words(unsigned long long seed, size_t n, std::vector<unsigned int>& out);
+extern "C" const int sdesim_jk_dims;  // dimensions of the embedded Joe-Kuo table
 
 namespace {
 constexpr int kNF = 40, kF = 3, kSeg = 16, kNQ = 2 + kNF;
@@ -72,6 +73,8 @@ int sdesim_lmm_create(const sdesim_lmm_spec* spec, sdesim_lmm** out) {
       throw LFail{SDESIM_EVALUE, "steps must equal steps_per_period * expiry_index"};
     if (s.paths < 1 || s.paths > (1LL << 22)) throw LFail{SDESIM_EVALUE, "paths must be in [1, 2^22]"};
     if (!(s.tenor > 0.0)) throw LFail{SDESIM_EVALUE, "tenor must be > 0"};
+    if ((long long)s.steps * kF > sdesim_jk_dims)
+      throw LFail{SDESIM_EVALUE, "Sobol dimension (steps x factors) exceeds " + std::to_string(sdesim_jk_dims)};
     if (s.storage < SDESIM_STORAGE_AUTO || s.storage > SDESIM_STORAGE_CHECKPOINTS)
       throw LFail{SDESIM_EVALUE, "storage must be SDESIM_STORAGE_AUTO, _TAPE or _CHECKPOINTS"};
     std::unique_ptr<sdesim_lmm> m(new sdesim_lmm());

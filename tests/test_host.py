@@ -466,7 +466,8 @@ def test_abi_argument_validation():
     for kw, msg in [(dict(forwards=20), "compiled for 40 forwards"), (dict(expiry_index=40), "expiry_index out of range"),
                     (dict(steps=150), "steps must equal steps_per_period * expiry_index"),
                     (dict(paths=(1 << 22) + 1), "paths must be in [1, 2^22]"), (dict(tenor=0.0), "tenor must be > 0"),
-                    (dict(storage=-1), "storage must be")]:
+                    (dict(storage=-1), "storage must be"),
+                    (dict(steps=7100, steps_per_period=1775, expiry_index=4), "Sobol dimension (steps x factors)")]:
         assert lmm_rc(**kw) == nat.EVALUE, kw
         assert msg in nat.last_error(), (kw, nat.last_error())
 
