@@ -182,17 +182,25 @@ class Plan:
         import torch
         return torch.empty(self.shape, dtype=self.dtype, device=device)
 
+    def _check_filtration(self, filtration):
+        if (tuple(filtration.shape) != self.shape or filtration.dtype != self.dtype or not filtration.is_contiguous()
+                or filtration.device.type != "cuda"):
+            raise ValueError("filtration must be a contiguous %s device tensor of shape %s" % (self.dtype, self.shape))
+
     def run(self, filtration=None, injected=None, stream=None):
         """Launch on `stream` (default: torch's current stream). Asynchronous."""
-        fptr = ctypes.c_void_p(filtration.data_ptr()) if filtration is not None else None
+        fptr = None
         if filtration is not None:
-            if tuple(filtration.shape) != self.shape or filtration.dtype != self.dtype or not filtration.is_contiguous():
-                raise ValueError("filtration must be a contiguous %s tensor of shape %s" % (self.dtype, self.shape))
+            self._check_filtration(filtration)
+            fptr = ctypes.c_void_p(filtration.data_ptr())
         iptr = None
         if injected is not None:
+            import torch
             n = (len(self.times) - 1) * self.universe.num_increments
-            if tuple(injected.shape) != (n, self.scenarios) or not injected.is_contiguous():
-                raise ValueError("injected uniforms must be contiguous [N*m][S] = %s" % ((n, self.scenarios),))
+            if (tuple(injected.shape) != (n, self.scenarios) or not injected.is_contiguous()
+                    or injected.dtype != torch.float64 or injected.device.type != "cuda"):
+                raise ValueError("injected uniforms must be a contiguous float64 device tensor [N*m][S] = %s"
+                                 % ((n, self.scenarios),))
             iptr = ctypes.c_void_p(injected.data_ptr())
         rc = nat.lib.sdesim_plan_run(self._h, fptr, iptr, _stream_handle(stream))
         nat.check(rc, runtime_prefix="Simulation failed: ")
@@ -209,6 +217,7 @@ class Plan:
     def to_long(self, filtration, stream=None):
         """Device [T][P][S] -> device [S][T][P] float64 (the long layout's value column)."""
         import torch
+        self._check_filtration(filtration)
         T, P, S = self.shape
         out = torch.empty((S, T, P), dtype=torch.float64, device=filtration.device)
         rc = nat.lib.sdesim_filtration_to_long(ctypes.c_void_p(filtration.data_ptr()), self.options.precision, S, T, P,

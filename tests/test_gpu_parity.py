@@ -522,3 +522,30 @@ def test_fp32_factored_coefficients(oracle, scheme, rng, scramble):
     ref = oracle.simulate(FACTORED, times, S, init, scheme=scheme, rng=rng, seed=17, scramble=scramble,
                           scramble_seed=5)
     _close(f.transpose(2, 0, 1), ref, rtol=5e-5, atol=1e-5)
+
+
+def test_plan_refuses_host_and_misshaped_tensors():
+    """Tensors handed to a plan are checked before any launch: a host tensor, a wrong shape or
+    dtype, or a non-contiguous view raises ValueError (for run, the injected uniforms and
+    to_long), and the plan still runs afterwards."""
+    import sde_sim_rs
+    times = np.linspace(0.0, 1.0, 9)
+    print("case: plan tensor checks", flush=True)
+    plan = sde_sim_rs.Plan(GBM, times, 100, {"X": 100.0}, "injected", "euler")
+    good = plan.new_filtration()
+    inj = torch.rand((8, 100), dtype=torch.float64, device="cuda")
+    bad = [torch.empty(plan.shape, dtype=torch.float64),                               # host memory
+           torch.empty((9, 1, 99), dtype=torch.float64, device="cuda"),                # shape
+           torch.empty(plan.shape, dtype=torch.float32, device="cuda"),                # dtype
+           torch.empty((100, 1, 9), dtype=torch.float64, device="cuda").permute(2, 1, 0)]  # layout
+    for f in bad:
+        with pytest.raises(ValueError):
+            plan.run(f, injected=inj)
+        with pytest.raises(ValueError):
+            plan.to_long(f)
+    for u in (inj.cpu(), inj.float(), inj[:, :99], inj.t().contiguous().t()):
+        with pytest.raises(ValueError):
+            plan.run(good, injected=u)
+    plan.run(good, injected=inj)
+    torch.cuda.synchronize()
+    assert torch.isfinite(good).all()
