@@ -95,11 +95,19 @@ class Basket:
 
     def evaluate(self, x0, mu, sigma, rho, weights=None, adjoint=True, stream=None):
         D = self.assets
-        x0 = np.ascontiguousarray(x0, dtype=np.float64)
-        mu = np.ascontiguousarray(mu, dtype=np.float64)
-        sigma = np.ascontiguousarray(sigma, dtype=np.float64)
-        w = np.full(D, 1.0 / D) if weights is None else np.ascontiguousarray(weights, dtype=np.float64)
-        L = np.ascontiguousarray(np.linalg.cholesky(np.asarray(rho, dtype=np.float64)))
+
+        def vec(v, name):
+            v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+            if v.size != D:
+                raise ValueError("%s must hold %d values (one per asset), got %d" % (name, D, v.size))
+            return v
+
+        x0, mu, sigma = vec(x0, "x0"), vec(mu, "mu"), vec(sigma, "sigma")
+        w = np.full(D, 1.0 / D) if weights is None else vec(weights, "weights")
+        rho = np.asarray(rho, dtype=np.float64)
+        if rho.shape != (D, D):
+            raise ValueError("rho must be a %d x %d correlation matrix, got shape %s" % (D, D, rho.shape))
+        L = np.ascontiguousarray(np.linalg.cholesky(rho))
         nl = D * (D + 1) // 2
         out = np.zeros(2 + 3 * D + nl, dtype=np.float64)
         nat.check(nat.lib.sdesim_basket_eval(self._h, nat.dptr(x0), nat.dptr(mu), nat.dptr(sigma), nat.dptr(w),

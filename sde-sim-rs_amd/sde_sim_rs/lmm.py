@@ -66,9 +66,12 @@ class ForwardRateModel:
         self.storage = "tape" if nat.lib.sdesim_lmm_storage(h) == nat.STORAGE_TAPE else "checkpoints"
 
     def evaluate(self, L0, vol, loadings, adjoint=True, stream=None):
-        L0 = np.ascontiguousarray(L0, dtype=np.float64)
-        vol = np.ascontiguousarray(vol, dtype=np.float64)
-        B = np.ascontiguousarray(loadings, dtype=np.float64).reshape(NF, NFACT)
+        L0 = np.ascontiguousarray(L0, dtype=np.float64).reshape(-1)
+        vol = np.ascontiguousarray(vol, dtype=np.float64).reshape(-1)
+        B = np.ascontiguousarray(loadings, dtype=np.float64)
+        if L0.size != NF or vol.size != NF or B.size != NF * NFACT:
+            raise ValueError("L0 and vol must hold %d values and loadings %d x %d" % (NF, NF, NFACT))
+        B = B.reshape(NF, NFACT)
         out = np.zeros(2 + NF, dtype=np.float64)
         nat.check(nat.lib.sdesim_lmm_eval(self._h, nat.dptr(L0), nat.dptr(vol), nat.dptr(B), 1 if adjoint else 0,
                                           nat.dptr(out), _stream_handle(stream)))

@@ -183,3 +183,19 @@ def test_handles_release_device_memory():
         cycle()
     after = torch.cuda.mem_get_info()[0]
     assert after >= before - (16 << 20), (before, after)
+
+
+def test_basket_refuses_mismatched_parameters():
+    """Parameter vectors of the wrong length and a correlation matrix of the wrong shape raise
+    ValueError before the library reads them."""
+    from sde_sim_rs.adjoint import Basket
+    print("case: basket parameter checks", flush=True)
+    b = Basket(3, 4, 256, seed=1, precision="f32")
+    ok = dict(x0=[100.0] * 3, mu=[0.01] * 3, sigma=[0.2] * 3, rho=np.eye(3))
+    for k, v in [("x0", [100.0] * 2), ("mu", [0.01] * 4), ("sigma", [0.2]), ("rho", np.eye(2))]:
+        with pytest.raises(ValueError):
+            b.evaluate(**dict(ok, **{k: v}))
+    with pytest.raises(ValueError):
+        b.evaluate(**ok, weights=[0.5, 0.5])
+    r = b.evaluate(**ok)
+    assert np.isfinite(r["price"]) and r["x0"].shape == (3,)

@@ -108,3 +108,19 @@ def test_lmm_tape_equals_checkpoints(precision):
     t, k = out["tape"], out["checkpoints"]
     assert abs(t["price"] - k["price"]) <= rtol * abs(k["price"])
     np.testing.assert_allclose(t["dL0"], k["dL0"], rtol=0, atol=rtol * np.max(np.abs(k["dL0"])))
+
+
+def test_forward_rate_model_refuses_mismatched_parameters():
+    """A curve, volatility vector or loading matrix of the wrong size raises ValueError before
+    the library reads it."""
+    from sde_sim_rs.lmm import ForwardRateModel, forward_rate_config
+    c = forward_rate_config(seed=5)
+    print("case: forward-rate parameter checks", flush=True)
+    m = ForwardRateModel(steps=16, paths=256, expiry_index=4, precision="f32")
+    for k, v in [("L0", c["L0"][:39]), ("vol", np.append(c["vol"], 0.2)), ("loadings", c["loadings"][:, :2])]:
+        args = dict(L0=c["L0"], vol=c["vol"], loadings=c["loadings"])
+        args[k] = v
+        with pytest.raises(ValueError):
+            m.evaluate(**args)
+    r = m.evaluate(c["L0"], c["vol"], c["loadings"])
+    assert np.isfinite(r["price"]) and r["dL0"].shape == (40,)
