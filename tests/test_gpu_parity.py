@@ -549,3 +549,27 @@ def test_plan_refuses_host_and_misshaped_tensors():
     plan.run(good, injected=inj)
     torch.cuda.synchronize()
     assert torch.isfinite(good).all()
+
+
+@pytest.mark.parametrize("precision,rtol", [("f64", 1e-12), ("f32", 2e-6)])
+def test_universes_without_stochastic_terms(oracle, precision, rtol):
+    """Universes with no stochastic increment run without a generator: a deterministic ODE, an
+    algebraic-only universe (its value at t0 is the initial row's 0, as the reference leaves it),
+    an ODE with an algebraic process, under Euler against the oracle; an empty equation list
+    gives an empty table; Runge-Kutta without a stochastic increment panics as in the reference
+    (src/sim/runge_kutta.rs:18)."""
+    import sde_sim_rs
+    times = np.linspace(0.0, 1.0, 11)
+    S = 300
+    cases = [(["dX = (0.1 * X) * dt + (0.5 * (1 - X)) * dt"], {"X": 2.0}),
+             (["Y = 2 * t + 1"], {}),
+             (["dX = (0.1 * X) * dt", "Y = X * t"], {"X": 1.0})]
+    for eqs, init in cases:
+        print("case: no stochastic terms %s %s" % (precision, eqs), flush=True)
+        _, f = _filtration(eqs, times, S, init, "pseudo", "euler", seed=1, precision=precision)
+        ref = oracle.simulate(eqs, times, S, init)
+        _close(f.transpose(2, 0, 1), ref, rtol=rtol, atol=1e-12)
+    table = sde_sim_rs.simulate([], list(times), 4, {}, "pseudo", "euler", precision=precision)
+    assert table.num_rows == 0 if hasattr(table, "num_rows") else len(table) == 0
+    with pytest.raises(sde_sim_rs.PanicException):
+        sde_sim_rs.simulate(cases[0][0], list(times), 4, {"X": 2.0}, "pseudo", "runge-kutta", precision=precision)
