@@ -1,7 +1,7 @@
 """Python handle on the oracle (test infrastructure only).
 
 liboracle.so is the CPU restatement of the reference simulator (sdesim_oracle.cpp). Only
-tests/, the smoke check and the benchmark's CPU-baseline leg use it, as the checker. The Sobol
+tests/, the smoke check and the benchmark's CPU-baseline leg use it, to compare against. The Sobol
 direction-number table is loaded from the copy scipy ships (Joe & Kuo, new-joe-kuo-6.21201).
 """
 import ctypes

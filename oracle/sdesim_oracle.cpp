@@ -1,6 +1,6 @@
 // sdesim_oracle.cpp -- CPU restatement of the sde-sim-rs simulator (TEST INFRASTRUCTURE ONLY).
 //
-// This file is the oracle: the checker the GPU product is compared against, and the timed CPU
+// This file is the oracle: the reference the GPU product is compared against, and the timed CPU
 // baseline. It is never linked into, loaded by or called from the product library. Only the
 // tests, the smoke check and the benchmark's CPU-baseline leg load it (through ctypes).
 //
