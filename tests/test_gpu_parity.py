@@ -501,26 +501,27 @@ FACTORED = ["dX = (0.1 * X) * dt + (0.3 * X) * dW1 + (-0.2 * X) * dW2",
 
 
 @pytest.mark.parametrize("rng,scramble", [("pseudo", "shift"), ("sobol", "shift"), ("sobol", "xor")])
-@pytest.mark.parametrize("scheme", ["euler", "runge-kutta"])
-def test_fp32_factored_coefficients(oracle, scheme, rng, scramble):
+@pytest.mark.parametrize("scheme,compat", [("euler", "reference"), ("runge-kutta", "reference"),
+                                           ("euler", "textbook"), ("runge-kutta", "textbook")])
+def test_fp32_factored_coefficients(oracle, scheme, compat, rng, scramble):
     """FP32 kernels move constant factors from coefficients to increments. In this universe:
     * X's Wiener terms form the group X * (0.3 dW1 - 0.2 dW2);
     * Y's form X * (0.125 dW2 + 0.4 dW3 + 2 dW3);
     * dW1 is used only with 0.3 > 0, so it is formed as 0.3 sqrt(dt) Psi(u) (and its uniform still
       gives the RK sign);
     * dW2 (two constants, one < 0) and dW3 (two constants) are not.
-    The paths follow the oracle's FP64 paths to FP32 accuracy: 3000 paths, 64 steps, every value
+    Both schemes, both cache modes, all three uniform generators. The paths follow the oracle's FP64 paths to FP32 accuracy: 3000 paths, 64 steps, every value
     within 5e-5 relative + 1e-5. The paths reach |value| ~ 11, where one FP32 ulp is ~1e-6, and Y
     passes through 0 by cancellation of O(1) increments, so an absolute term of ~10 ulps at the
     paths' scale is FP32 rounding; a wrong constant or scaling errs by 1e-2 or more."""
     times = np.linspace(0.0, 1.0, 65)
     S = 3000
     init = {"X": 1.0, "Y": 0.5}
-    print("case: fp32 factored %s %s/%s S=%d" % (scheme, rng, scramble, S), flush=True)
+    print("case: fp32 factored %s %s %s/%s S=%d" % (scheme, compat, rng, scramble, S), flush=True)
     _, f = _filtration(FACTORED, times, S, init, rng, scheme, seed=17, scramble=scramble, scramble_seed=5,
-                       precision="f32")
+                       precision="f32", compat=compat)
     ref = oracle.simulate(FACTORED, times, S, init, scheme=scheme, rng=rng, seed=17, scramble=scramble,
-                          scramble_seed=5)
+                          scramble_seed=5, compat=compat)
     _close(f.transpose(2, 0, 1), ref, rtol=5e-5, atol=1e-5)
 
 
