@@ -574,3 +574,46 @@ def test_universes_without_stochastic_terms(oracle, precision, rtol):
     assert table.num_rows == 0 if hasattr(table, "num_rows") else len(table) == 0
     with pytest.raises(sde_sim_rs.PanicException):
         sde_sim_rs.simulate(cases[0][0], list(times), 4, {"X": 2.0}, "pseudo", "runge-kutta", precision=precision)
+
+
+def _smooth_coef(rng, names, depth):
+    """A random smooth expression over the process names and t (sums, products, sin, cos, tanh,
+    atan): no kinks, so FP32 rounding stays at FP32 size through the paths."""
+    if depth == 0 or rng.random() < 0.3:
+        c = rng.integers(0, 3)
+        return str(rng.choice(names)) if c == 0 else ("t" if c == 1 else "%.4g" % rng.uniform(0.1, 1.5))
+    a, b = _smooth_coef(rng, names, depth - 1), _smooth_coef(rng, names, depth - 1)
+    return ["(%s + %s)" % (a, b), "(%s * %s)" % (a, b), "sin(%s)" % a, "cos(%s)" % a, "tanh(%s)" % a,
+            "atan(%s)" % a][int(rng.integers(0, 6))]
+
+
+def test_random_factored_universes_fp32(oracle):
+    """Random equation sets built to exercise the FP32 constant factoring: each process has a dt
+    term c0 * r0 and 2-3 Wiener terms c_k * r over one shared residual r, with random constants
+    of either sign and random increments from dW1-dW3 (shared across processes, sometimes
+    repeated within one). Random scheme, cache mode and generator; FP32 paths against the
+    oracle's FP64 paths, every value within 1e-4 relative + 1e-5 times the largest |value|."""
+    rng = np.random.default_rng(44)
+    times = np.linspace(0.0, 1.0, 17)
+    for case in range(10):
+        P = int(rng.integers(1, 3))
+        names = ["X", "Y"][:P]
+        eqs = []
+        for n in names:
+            terms = ["(%.3g * %s) * dt" % (rng.uniform(-0.5, 0.5), _smooth_coef(rng, names, 2))]
+            r = _smooth_coef(rng, names, 2)
+            for _ in range(int(rng.integers(2, 4))):
+                terms.append("(%.3g * %s) * dW%d" % (rng.choice([-1, 1]) * rng.uniform(0.05, 0.4), r,
+                                                    int(rng.integers(1, 4))))
+            eqs.append("d%s = %s" % (n, " + ".join(terms)))
+        scheme = str(rng.choice(["euler", "runge-kutta"]))
+        compat = str(rng.choice(["reference", "textbook"]))
+        rng_method, scramble = [("pseudo", "shift"), ("sobol", "shift"), ("sobol", "xor")][int(rng.integers(0, 3))]
+        init = {n: float(rng.uniform(0.5, 1.5)) for n in names}
+        seed = int(rng.integers(0, 1000))
+        print("case %d: %s %s %s/%s %s" % (case, scheme, compat, rng_method, scramble, eqs), flush=True)
+        _, f = _filtration(eqs, times, 300, init, rng_method, scheme, seed=seed, scramble=scramble,
+                           scramble_seed=seed + 1, precision="f32", compat=compat)
+        ref = oracle.simulate(eqs, times, 300, init, scheme=scheme, rng=rng_method, seed=seed, scramble=scramble,
+                              scramble_seed=seed + 1, compat=compat)
+        _close(f.transpose(2, 0, 1), ref, rtol=1e-4, atol=1e-5 * max(1.0, float(np.abs(ref).max())))
