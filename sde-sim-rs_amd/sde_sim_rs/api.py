@@ -243,8 +243,9 @@ def long_table(names, times, scenarios, values, chunk_rows=1 << 24):
 
     The columns are built in chunks of whole scenarios (about `chunk_rows` rows each), so no
     column is materialised at once beside the values and no string chunk reaches Arrow's 2^31-byte
-    limit (a 2^20-path, 1025-step, 2-process run has 2.15e9 rows). `values` ([S][T][P] f64) is
-    wrapped without a copy."""
+    limit (a 2^20-path, 1025-step, 2-process run has 2.15e9 rows). The time and process_name
+    chunks repeat from one chunk to the next, so each is built once and shared (a shorter last
+    chunk is a zero-copy prefix); `values` ([S][T][P] f64) is wrapped without a copy."""
     import pyarrow as pa
     names = list(names)
     T, P, S = len(times), len(names), int(scenarios)
@@ -255,15 +256,18 @@ def long_table(names, times, scenarios, values, chunk_rows=1 << 24):
     longest = max([len(n.encode("utf-8")) for n in names] + [1])
     rows = max(1, min(int(chunk_rows), ((1 << 31) - 1) // longest))
     spc = max(1, rows // max(1, per))  # scenarios per chunk
+    spc = min(spc, max(1, S))
     tpat = np.repeat(np.asarray(times, dtype=np.float64), P)
     dictionary = pa.array(names, type=pa.string())
+    time_chunk = pa.array(np.tile(tpat, spc))
+    idx = pa.array(np.tile(np.arange(P, dtype=np.int32), spc * T))
+    name_chunk = pa.DictionaryArray.from_arrays(idx, dictionary).dictionary_decode()
     cols = {"scenario": [], "time": [], "process_name": [], "value": []}
     for s0 in range(0, S, spc):
         n = min(spc, S - s0)
         cols["scenario"].append(pa.array(np.repeat(np.arange(s0, s0 + n, dtype=np.int32), per)))
-        cols["time"].append(pa.array(np.tile(tpat, n)))
-        idx = pa.array(np.tile(np.arange(P, dtype=np.int32), n * T))
-        cols["process_name"].append(pa.DictionaryArray.from_arrays(idx, dictionary).dictionary_decode())
+        cols["time"].append(time_chunk if n == spc else time_chunk.slice(0, n * per))
+        cols["process_name"].append(name_chunk if n == spc else name_chunk.slice(0, n * per))
         cols["value"].append(pa.array(vals[s0 * per:(s0 + n) * per]))
     types = {"scenario": pa.int32(), "time": pa.float64(), "process_name": pa.string(), "value": pa.float64()}
     table = pa.table({k: pa.chunked_array(v, type=types[k]) for k, v in cols.items()})
