@@ -1,8 +1,8 @@
 // basket_kernels.hip -- correlated GBM basket: price and pathwise adjoint (reverse sweep).
 //
-// Model: D assets, X_{n+1,i} = X_{n,i} + X_{n,i} (mu_i dt + sigma_i sqrt(dt) Z_{n,i}),
+// Model: A assets, X_{n+1,i} = X_{n,i} + X_{n,i} (mu_i dt + sigma_i sqrt(dt) Z_{n,i}),
 // Z_n = L eps_n with L the lower Cholesky factor of the correlation and eps_{n,k} = Psi(u_{n,k})
-// (A&S inverse normal, src/proc/increment.rs:160-179) of the pseudo-random draw n*D + k of
+// (A&S inverse normal, src/proc/increment.rs:160-179) of the pseudo-random draw n*A + k of
 // ChaCha8Rng::seed_from_u64(seed + s) -- the Euler step of the reference (src/sim/euler.rs:5-37)
 // applied to the DSL form dX_i = (mu_i * X_i) * dt + sum_k (sigma_i * L_ik * X_i) * dW_k.
 // Payoff on the weighted basket B = sum_i w_i X_{N,i}: width * softplus((B - K) / width), or the
@@ -39,10 +39,33 @@ struct BasketConst {
 };
 
 // The words (low, high) of the D f64 draws of step n: draws n*D .. n*D + D - 1 of the path's
-// stream (draw j is words 2j and 2j + 1).
-template <int D>
-__device__ __forceinline__ void basket_draws(const sde_u32 (&key)[8], int n, sde_u32 (&lo)[D], sde_u32 (&hi)[D]) {
-  if constexpr (D == 8) {
+// stream (draw j is words 2j and 2j + 1). PADDED (D = 8 holding A < 8 assets, the rest padding
+// with zero weight): step n uses draws n*A .. n*A + A - 1, as the DSL form with A Wiener
+// increments does; they lie in ChaCha blocks (n*A) / 8 and the next one, and the wave-uniform
+// word offset (n*A) mod 8 selects them (the padding assets take the following draws, unused).
+template <int D, bool PADDED>
+__device__ __forceinline__ void basket_draws(const sde_u32 (&key)[8], int n, int A, sde_u32 (&lo)[D],
+                                             sde_u32 (&hi)[D]) {
+  if constexpr (PADDED) {
+    static_assert(D == 8, "padded baskets run the 8-asset kernels");
+    const long long j0 = (long long)n * A;
+    sde_u32 b0[16], b1[16];
+    sde_chacha_block<8>(key, (sde_u64)(j0 >> 3), b0);
+    sde_chacha_block<8>(key, (sde_u64)(j0 >> 3) + 1, b1);
+    const int w0 = (int)(j0 & 7);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int w = w0 + k;  // < 15
+      sde_u32 l = b0[0], h = b0[1];
+#pragma unroll
+      for (int q = 1; q < 16; ++q) {
+        l = (w == q) ? (q < 8 ? b0[2 * q] : b1[2 * q - 16]) : l;
+        h = (w == q) ? (q < 8 ? b0[2 * q + 1] : b1[2 * q - 15]) : h;
+      }
+      lo[k] = l;
+      hi[k] = h;
+    }
+  } else if constexpr (D == 8) {
     sde_u32 blk[16];
     sde_chacha_block<8>(key, (sde_u64)n, blk);
 #pragma unroll
@@ -74,11 +97,11 @@ struct BasketStep {
   R eps[D], Z[D];
 };
 
-template <typename R, int D>
-__device__ __forceinline__ void basket_noise(const sde_u32 (&key)[8], int n, const R (&L)[D][D],
+template <typename R, int D, bool PADDED>
+__device__ __forceinline__ void basket_noise(const sde_u32 (&key)[8], int n, int A, const R (&L)[D][D],
                                              BasketStep<R, D>& st) {
   sde_u32 lo[D], hi[D];
-  basket_draws<D>(key, n, lo, hi);
+  basket_draws<D, PADDED>(key, n, A, lo, hi);
 #pragma unroll
   for (int k = 0; k < D; ++k) st.eps[k] = sde_psi_words<R>(lo[k], hi[k]);
 #pragma unroll
@@ -199,7 +222,7 @@ struct BasketReverse {
 // MODE 0: price; 1: price and adjoint with checkpoints and recomputation; 2: the forward pass of
 // the tape form, which writes X_0 .. X_N to the tape and leaves the payoff, the reverse sweep and
 // every partial to sdesim_basket_reverse.
-template <typename R, int D, int SEG, int MODE>
+template <typename R, int D, int SEG, int MODE, bool PADDED>
 __global__ void __launch_bounds__(256) sdesim_basket(const BasketArgs a) {
   constexpr bool TAPE = MODE == 2;
   constexpr int NQ = BasketConst<D>::NQ;
@@ -236,7 +259,7 @@ __global__ void __launch_bounds__(256) sdesim_basket(const BasketArgs a) {
       for (int i = 0; i < D; ++i) ck[(slot * D + i) * S + sid] = X[i];
     }
     BasketStep<R, D> st;
-    basket_noise<R, D>(key, n, L, st);
+    basket_noise<R, D, PADDED>(key, n, a.assets, L, st);
 #pragma unroll
     for (int i = 0; i < D; ++i) X[i] = X[i] + X[i] * (mu[i] * dt + sig[i] * sdt * st.Z[i]);
   }
@@ -269,7 +292,7 @@ __global__ void __launch_bounds__(256) sdesim_basket(const BasketArgs a) {
         for (int i = 0; i < D; ++i) Xs[k][i] = Xc[i];
         if (n0 + k < N && k + 1 < SEG) {
           BasketStep<R, D> st;
-          basket_noise<R, D>(key, n0 + k, L, st);
+          basket_noise<R, D, PADDED>(key, n0 + k, a.assets, L, st);
 #pragma unroll
           for (int i = 0; i < D; ++i) Xc[i] = Xc[i] + Xc[i] * (mu[i] * dt + sig[i] * sdt * st.Z[i]);
         }
@@ -355,24 +378,33 @@ __global__ void __launch_bounds__(256) sdesim_sum_partials(const double* __restr
 // 2 adjoint with the tape (a.states holds [steps + 1][D][paths]; a forward and a reverse kernel).
 hipError_t sdesim_launch_basket(int f32, int assets, int mode, int grid, const BasketArgs& a, int nq,
                                 double* sums, hipStream_t stream) {
-#define SDE_BASKET(R, D, MODE)                                                                             \
-  hipLaunchKernelGGL((sdesim_basket<R, D, 19, MODE>), dim3(grid), dim3(256), 0, stream, a);                   \
+  // assets 1 runs the one-asset kernels, 8 the eight-asset ones, and 2-7 the eight-asset ones
+  // padded (their own draw stride).
+#define SDE_BASKET(R, D, MODE, PAD)                                                                        \
+  hipLaunchKernelGGL((sdesim_basket<R, D, 19, MODE, PAD>), dim3(grid), dim3(256), 0, stream, a);              \
   if (MODE == 2) hipLaunchKernelGGL((sdesim_basket_reverse<R, D>), dim3(grid), dim3(256), 0, stream, a);     \
   break;
   bool ok = true;
-  switch ((f32 ? 100 : 0) + assets * 3 + mode) {
-    case 100 + 24 + 2: SDE_BASKET(float, 8, 2)
-    case 100 + 24 + 1: SDE_BASKET(float, 8, 1)
-    case 100 + 24 + 0: SDE_BASKET(float, 8, 0)
-    case 100 + 3 + 2: SDE_BASKET(float, 1, 2)
-    case 100 + 3 + 1: SDE_BASKET(float, 1, 1)
-    case 100 + 3 + 0: SDE_BASKET(float, 1, 0)
-    case 24 + 2: SDE_BASKET(double, 8, 2)
-    case 24 + 1: SDE_BASKET(double, 8, 1)
-    case 24 + 0: SDE_BASKET(double, 8, 0)
-    case 3 + 2: SDE_BASKET(double, 1, 2)
-    case 3 + 1: SDE_BASKET(double, 1, 1)
-    case 3 + 0: SDE_BASKET(double, 1, 0)
+  const int shape = assets == 1 ? 1 : assets == 8 ? 8 : (assets >= 2 && assets < 8) ? 9 : 0;
+  switch ((f32 ? 100 : 0) + shape * 3 + mode) {
+    case 100 + 27 + 2: SDE_BASKET(float, 8, 2, true)
+    case 100 + 27 + 1: SDE_BASKET(float, 8, 1, true)
+    case 100 + 27 + 0: SDE_BASKET(float, 8, 0, true)
+    case 100 + 24 + 2: SDE_BASKET(float, 8, 2, false)
+    case 100 + 24 + 1: SDE_BASKET(float, 8, 1, false)
+    case 100 + 24 + 0: SDE_BASKET(float, 8, 0, false)
+    case 100 + 3 + 2: SDE_BASKET(float, 1, 2, false)
+    case 100 + 3 + 1: SDE_BASKET(float, 1, 1, false)
+    case 100 + 3 + 0: SDE_BASKET(float, 1, 0, false)
+    case 27 + 2: SDE_BASKET(double, 8, 2, true)
+    case 27 + 1: SDE_BASKET(double, 8, 1, true)
+    case 27 + 0: SDE_BASKET(double, 8, 0, true)
+    case 24 + 2: SDE_BASKET(double, 8, 2, false)
+    case 24 + 1: SDE_BASKET(double, 8, 1, false)
+    case 24 + 0: SDE_BASKET(double, 8, 0, false)
+    case 3 + 2: SDE_BASKET(double, 1, 2, false)
+    case 3 + 1: SDE_BASKET(double, 1, 1, false)
+    case 3 + 0: SDE_BASKET(double, 1, 0, false)
     default: ok = false;
   }
 #undef SDE_BASKET

@@ -83,7 +83,7 @@ struct BasketArgs {
                               // [slots][D][paths] real (one slot per 19 steps)
   double* partials;           // [gridDim.x][NQ]
   int steps;
-  int pad_;
+  int assets;                 // the basket's asset count A: step n uses draws n * A .. n * A + A - 1
 };
 
 // ------------------------------------------------------------------------------------------

@@ -101,18 +101,9 @@ std::map<std::pair<int, std::string>, CompiledKernel> g_kernels;
 const char* const kRtcOptions[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=fast",
                                    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"};
 
-// The device header the run-time kernels compile against. Kernel development:
-// SDESIM_DEVICE_HEADER names a header file to use instead of the copy embedded at build time.
+// The device header the run-time kernels compile against: the copy embedded at build time.
 const std::string& device_header_text() {
-  static const std::string text = [] {
-    const char* path = std::getenv("SDESIM_DEVICE_HEADER");
-    if (!path || !*path) return std::string(sdesim_embedded_device_header);
-    std::ifstream in(path);
-    if (!in) return std::string(sdesim_embedded_device_header);
-    std::stringstream ss;
-    ss << in.rdbuf();
-    return ss.str();
-  }();
+  static const std::string text(sdesim_embedded_device_header);
   return text;
 }
 

@@ -131,9 +131,10 @@ int sdesim_basket_eval(sdesim_basket* b, const double* x0, const double* mu, con
     a.steps = s.steps;
     a.states = b->states;
     a.partials = b->partials;
+    a.assets = A;
     hipStream_t st = (hipStream_t)stream;
     const int mode = !adjoint ? 0 : b->storage == SDESIM_STORAGE_TAPE ? 2 : 1;
-    bk(sdesim_launch_basket(s.precision == SDESIM_F32 ? 1 : 0, D, mode, (int)b->grid, a, b->nq, b->sums, st),
+    bk(sdesim_launch_basket(s.precision == SDESIM_F32 ? 1 : 0, A, mode, (int)b->grid, a, b->nq, b->sums, st),
        "basket launch");
     std::vector<double> sums(b->nq);
     bk(hipMemcpyAsync(sums.data(), b->sums, sizeof(double) * b->nq, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");

@@ -156,7 +156,12 @@ class Plan:
         if isinstance(initial_values, Mapping):
             self.x0 = universe.initial_row(initial_values)
         else:
-            self.x0 = np.ascontiguousarray(np.asarray(initial_values, dtype=np.float64))
+            # One value per process, in registry order: the library copies exactly that many.
+            x0 = np.ascontiguousarray(np.asarray(initial_values, dtype=np.float64))
+            if x0.ndim != 1 or x0.size != universe.num_processes:
+                raise ValueError("initial_values must be a mapping of process names or a sequence of %d values "
+                                 "(one per process), got shape %s" % (universe.num_processes, x0.shape))
+            self.x0 = x0 if x0.size else np.zeros(1, dtype=np.float64)
         h = ctypes.c_void_p()
         rc = nat.lib.sdesim_plan_create(universe._h, nat.dptr(self.times), len(self.times), nat.dptr(self.x0),
                                         self.scenarios, ctypes.byref(self.options),

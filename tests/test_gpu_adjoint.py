@@ -31,11 +31,36 @@ def test_basket_price_matches_dsl_form():
     assert abs(r["price"] - m[0]) <= 1e-9 * abs(m[0]), (r["price"], m[0])
 
 
-def test_adjoint_f64_matches_torch_autodiff(oracle):
+@pytest.mark.parametrize("assets", [3, 5])
+def test_basket_fewer_than_eight_assets_matches_dsl_form(assets):
+    """Baskets of 2-7 assets run the eight-asset kernels padded: asset k of step n still draws
+    number n*A + k of the path's stream, as the A-increment DSL form does, so the price equals the
+    DSL form's (FP64, plain call)."""
+    import sde_sim_rs
+    from sde_sim_rs.adjoint import Basket, basket_dsl
+    c = _cfg()
+    A = assets
+    x0, mu, sig, rho = c["x0"][:A], c["mu"][:A], c["sigma"][:A], c["rho"][:A, :A]
+    S, N = 1 << 13, 29
+    print("case: basket f64 price vs DSL form A=%d S=%d N=%d" % (A, S, N), flush=True)
+    b = Basket(A, N, S, horizon=1.0, strike=100.0, width=0.0, seed=23, precision="f64")
+    r = b.evaluate(x0, mu, sig, rho, adjoint=False)
+    eqs, init = basket_dsl(x0, mu, sig, rho)
+    times = np.arange(N + 1) * (1.0 / N)
+    plan = sde_sim_rs.Plan(eqs, times, S, init, "pseudo", "euler", seed=23, outputs=("terminal",),
+                           payoff="max((%s) / %d - 100, 0)" % (" + ".join("X%d" % i for i in range(A)), A))
+    plan.run(None)
+    m = plan.moments()["payoff"]
+    assert abs(r["price"] - m[0]) <= 1e-9 * abs(m[0]), (r["price"], m[0])
+
+
+@pytest.mark.parametrize("D", [8, 3])
+def test_adjoint_f64_matches_torch_autodiff(oracle, D):
     from sde_sim_rs.adjoint import Basket
     c = _cfg()
-    S, N, D, seed = 32, 45, 8, 1234
-    print("case: basket f64 adjoint vs torch autodiff S=%d N=%d" % (S, N), flush=True)
+    c = {"x0": c["x0"][:D], "mu": c["mu"][:D], "sigma": c["sigma"][:D], "rho": c["rho"][:D, :D]}
+    S, N, seed = 32, 45, 1234
+    print("case: basket f64 adjoint vs torch autodiff D=%d S=%d N=%d" % (D, S, N), flush=True)
     b = Basket(D, N, S, horizon=1.0, strike=100.0, width=0.5, seed=seed, precision="f64")
     r = b.evaluate(c["x0"], c["mu"], c["sigma"], c["rho"], adjoint=True)
     # Reference: the same paths in torch float64 on the CPU, differentiated by autograd.

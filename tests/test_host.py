@@ -509,3 +509,13 @@ def test_abi_argument_validation():
         precision, S, T, P = args
         assert to_long(None, precision, S, T, P, None, None) == nat.EVALUE, args
         assert msg in nat.last_error(), (args, nat.last_error())
+
+
+def test_plan_refuses_initial_values_of_the_wrong_length():
+    """A sequence of initial values must hold one value per process (registry order): the library
+    copies exactly that many, so a shorter or misshaped array is refused before any device call."""
+    u = sde_sim_rs.Universe(HESTON)
+    times = np.linspace(0.0, 1.0, 5)
+    for bad in ([100.0], [100.0, 0.04, 1.0], [[100.0, 0.04]], np.zeros((2, 1))):
+        with pytest.raises(ValueError, match="one per process"):
+            sde_sim_rs.Plan(u, times, 8, bad, "sobol", "runge-kutta")
