@@ -111,20 +111,35 @@ def _profiled_traffic(paths, nsteps):
     return None
 
 
-def cpu_baseline(paths, steps_n, threads, runs=3):
+def host_cpu():
+    """(model name, online CPUs, CPUs this process may run on)."""
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count() or 1, len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(paths, steps_n, threads, scramble="xor", runs=3):
     """The oracle (the reference simulator's CPU restatement) on `paths` paths of the headline
-    workload, `threads` threads; the median of `runs` timed runs after a small warm-up."""
+    workload, `threads` threads; the median of `runs` timed runs after a small warm-up.
+    scramble="shift" runs the reference's own randomisation (a per-path f64 random shift of the
+    Sobol points, src/rng/sobol.rs:62-79) instead of the digital XOR shift."""
     import numpy as np
 
     from oracle import oracle as O
 
     times = np.linspace(0.0, 1.0, steps_n + 1)
-    O.simulate(HESTON, times, 64, INIT, scheme="runge-kutta", rng="sobol", scramble="xor", scramble_seed=7,
+    O.simulate(HESTON, times, 64, INIT, scheme="runge-kutta", rng="sobol", scramble=scramble, scramble_seed=7,
                threads=threads)
     dts = []
     for _ in range(runs):
         t0 = time.perf_counter()
-        O.simulate(HESTON, times, paths, INIT, scheme="runge-kutta", rng="sobol", scramble="xor", scramble_seed=7,
+        O.simulate(HESTON, times, paths, INIT, scheme="runge-kutta", rng="sobol", scramble=scramble, scramble_seed=7,
                    threads=threads)
         dts.append(time.perf_counter() - t0)
     dt = sorted(dts)[runs // 2]
@@ -350,12 +365,22 @@ def main():
     if lmm:
         line["forward_rate_model"] = lmm
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        # Every CPU this process may use, up to the 16 that one GPU's job is allotted on a shared
+        # GPU host (a node's CPUs are divided among the jobs of its GPUs).
+        model, online, allowed = host_cpu()
+        threads = max(1, min(16, allowed))
         v, dts = cpu_baseline(args.cpu_paths, N, threads)
+        vs, dts_s = cpu_baseline(args.cpu_paths, N, threads, scramble="shift")
         line["cpu_baseline"] = {"value": v, "unit": "path-steps/s", "cores": threads, "kind": "port",
+                                "cpu_model": model, "cpus_online": online, "cpus_allowed": allowed,
                                 "sample": "same workload, %d paths x %d steps, oracle restatement of the reference "
-                                          "simulator (f64, full filtration); median of 3 runs (%s s)"
-                                          % (args.cpu_paths, N, ", ".join("%.1f" % d for d in dts))}
+                                          "simulator (f64, full filtration, Sobol-XOR); median of 3 runs (%s s)"
+                                          % (args.cpu_paths, N, ", ".join("%.1f" % d for d in dts)),
+                                "reference_shift_mode": {
+                                    "value": vs, "unit": "path-steps/s",
+                                    "sample": "the same paths with the reference's own per-path f64 random shift of "
+                                              "the Sobol points; median of 3 runs (%s s)"
+                                              % ", ".join("%.1f" % d for d in dts_s)}}
     print(json.dumps(line))
 
 

@@ -25,6 +25,7 @@ def lib():
         P = ctypes.POINTER
         L.oracle_chacha_block.argtypes = [P(ctypes.c_uint32), I32, P(ctypes.c_uint32)]
         L.oracle_seed_from_u64.argtypes = [U64, P(ctypes.c_uint32)]
+        L.oracle_pcg32.argtypes = [U64, U64, I32, P(ctypes.c_uint32)]
         L.oracle_pseudo_f64.argtypes = [U64, U64, I64, P(D)]
         L.oracle_sobol_init.argtypes = [I32, P(ctypes.c_int64), P(ctypes.c_int64)]
         L.oracle_sobol_dims.restype = I32
@@ -83,6 +84,13 @@ def seed_from_u64(seed):
     key = np.zeros(8, dtype=np.uint32)
     lib().oracle_seed_from_u64(seed, _u32p(key))
     return key
+
+
+def pcg32(state, inc, n):
+    """n outputs of the seed expansion's PCG32 step (advance, then XSH-RR of the new state)."""
+    out = np.zeros(n, dtype=np.uint32)
+    lib().oracle_pcg32(state & (2 ** 64 - 1), inc & (2 ** 64 - 1), n, _u32p(out))
+    return out
 
 
 def pseudo_f64(seed, n, first=0):

@@ -70,16 +70,19 @@ static void chacha_block(const uint32_t in[16], int rounds, uint32_t out[16]) {
   for (int i = 0; i < 16; ++i) out[i] = x[i] + in[i];
 }
 
-// rand_core::SeedableRng::seed_from_u64: PCG32 (MUL 6364136223846793005, INC
-// 11634580027462260723), one u32 per 4 seed bytes, little endian.
+// One PCG32 output as rand_core's seed_from_u64 draws it: advance the LCG (multiplier
+// 6364136223846793005, increment `inc`) first, then the XSH-RR output function of the new state.
+static uint32_t pcg32_advance_output(uint64_t& state, uint64_t inc) {
+  state = state * 6364136223846793005ULL + inc;
+  uint32_t xorshifted = (uint32_t)(((state >> 18) ^ state) >> 27);
+  uint32_t rot = (uint32_t)(state >> 59);
+  return (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
+}
+
+// rand_core::SeedableRng::seed_from_u64: PCG32 with increment 11634580027462260723, one u32 per
+// 4 seed bytes, little endian.
 static void seed_from_u64(uint64_t state, uint32_t key[8]) {
-  const uint64_t MUL = 6364136223846793005ULL, INC = 11634580027462260723ULL;
-  for (int i = 0; i < 8; ++i) {
-    state = state * MUL + INC;
-    uint32_t xorshifted = (uint32_t)(((state >> 18) ^ state) >> 27);
-    uint32_t rot = (uint32_t)(state >> 59);
-    key[i] = (xorshifted >> rot) | (xorshifted << ((32 - rot) & 31));
-  }
+  for (int i = 0; i < 8; ++i) key[i] = pcg32_advance_output(state, 11634580027462260723ULL);
 }
 
 // ChaCha8Rng::seed_from_u64(seed) as a u32 word stream; next_f64 = rand's StandardUniform.
@@ -753,6 +756,10 @@ extern "C" {
 
 void oracle_chacha_block(const uint32_t in[16], int rounds, uint32_t out[16]) { chacha_block(in, rounds, out); }
 void oracle_seed_from_u64(uint64_t seed, uint32_t key[8]) { seed_from_u64(seed, key); }
+// n outputs of pcg32_advance_output from `state` with increment `inc` (the seed expansion's step).
+void oracle_pcg32(uint64_t state, uint64_t inc, int n, uint32_t* out) {
+  for (int i = 0; i < n; ++i) out[i] = pcg32_advance_output(state, inc);
+}
 
 // n consecutive f64 draws of ChaCha8Rng::seed_from_u64(seed), starting at draw `first`.
 void oracle_pseudo_f64(uint64_t seed, uint64_t first, int64_t n, double* out) {

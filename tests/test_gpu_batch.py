@@ -48,20 +48,26 @@ def test_batch_moments_fp32_vs_fp64_and_closed_form():
     from sde_sim_rs import models
     insts = models.batched_instances(n=192, seed=11)
     print("case: batch 192 instances f32/f64 moments", flush=True)
-    m = {}
+    m, term = {}, {}
     for prec in ("f64", "f32"):
-        b = sde_sim_rs.Batch(insts, precision=prec)
+        b = sde_sim_rs.Batch(insts, precision=prec, keep_terminal=True)
         b.run()
         m[prec] = b.moments()
+        term[prec] = b.terminal().astype(np.float64)
+    off = 0
     for i, inst in enumerate(insts):
         mean64, var64 = m["f64"][i]
         mean32 = m["f32"][i][0]
         se = np.sqrt(var64 / inst["paths"])
+        d = term["f32"][off:off + inst["paths"]] - term["f64"][off:off + inst["paths"]]
+        off += inst["paths"]
         if inst["kind"] == models.CIR:
             # sigma * sqrt(max(X, 0)) is not Lipschitz at 0: once a path comes near zero its f32
-            # and f64 copies decorrelate, so the two means are two correlated MC estimates of the
-            # same value and the bound is statistical.
-            assert abs(mean32 - mean64) <= 2.0 * se + 1e-12, (i, inst, mean32, mean64, se)
+            # and f64 copies decorrelate, so the comparison is statistical: the mean of the
+            # per-path differences within 5 of its standard errors, plus the FP32 allowance of the
+            # other models (the f32 rounding of the parameters moves the mean itself).
+            se_d = d.std(ddof=1) / np.sqrt(d.size) if d.size > 1 else 0.0
+            assert abs(d.mean()) <= 5.0 * se_d + 1e-5 * (abs(mean64) + np.sqrt(var64)), (i, inst, d.mean(), se_d)
         else:
             assert abs(mean32 - mean64) <= 1e-5 * abs(mean64), (i, inst, mean32, mean64)
         if inst["kind"] in (models.GBM, models.OU):

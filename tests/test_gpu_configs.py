@@ -160,35 +160,51 @@ def test_c3_batch_against_oracle(oracle):
 
 
 def test_c3_full_batch_fp32_fp64_and_closed_forms():
-    """All 4096 instances: FP32 means within 1e-5 of the FP64 means, relative to the instance's
-    scale |mean| + sd (CIR: within 2 standard errors, since sqrt(max(X, 0)) decorrelates the FP32
-    and FP64 copies of a path near 0); GBM and OU FP64 means within 5 standard errors of the
-    Euler chain's exact mean."""
+    """All 4096 instances in FP32 and FP64 with every terminal value kept.
+
+    GBM and OU: the FP32 mean within 1e-5 of the FP64 mean, relative to the instance's scale
+    |mean| + sd, and the FP64 mean within 5 standard errors of the Euler chain's exact mean.
+    CIR: sqrt(max(X, 0)) is not Lipschitz at 0, so near 0 the FP32 and FP64 copies of a path
+    decorrelate (an instance with theta ~ 1e-4 differs on ~93 % of its paths) and the comparison
+    is statistical: the mean of the per-path differences x32 - x64 within 5 of its standard errors
+    (a paired test; 5 sigma over ~1400 instances) plus the FP32 allowance of the other models,
+    1e-5 (|mean| + sd) (where the copies stay together, the paired test resolves the f32 rounding
+    of the parameters themselves: theta ~ 0.09 moves the mean by ~3e-8 relative)."""
     import sde_sim_rs
     from sde_sim_rs import models
     insts = models.batched_instances(n=4096, seed=11)
-    print("case: c3 batch 4096 instances f32/f64 moments", flush=True)
-    m = {}
+    print("case: c3 batch 4096 instances f32/f64 terminal values", flush=True)
+    m, term = {}, {}
     for prec in ("f64", "f32"):
-        b = sde_sim_rs.Batch(insts, precision=prec)
+        b = sde_sim_rs.Batch(insts, precision=prec, keep_terminal=True)
         b.run()
         m[prec] = b.moments()
+        term[prec] = b.terminal().astype(np.float64)
         del b
-    worst = 0.0
+    offs = np.concatenate([[0], np.cumsum([x["paths"] for x in insts])]).astype(np.int64)
+    d = term["f32"] - term["f64"]
+    paths = np.diff(offs).astype(np.float64)
+    dmean = np.add.reduceat(d, offs[:-1]) / paths
+    dvar = (np.add.reduceat(d * d, offs[:-1]) - paths * dmean ** 2) / np.maximum(paths - 1.0, 1.0)
+    se_d = np.sqrt(np.maximum(dvar, 0.0) / paths)
+    worst = worst_cir = 0.0
     for i, x in enumerate(insts):
         mean64, var64 = m["f64"][i]
         mean32 = m["f32"][i][0]
         sd = np.sqrt(var64)
-        se = sd / np.sqrt(x["paths"])
         if x["kind"] == models.CIR:
-            assert abs(mean32 - mean64) <= 2.0 * se + 1e-12, (i, x, mean32, mean64, se)
+            z = abs(dmean[i]) / (se_d[i] + 1e-300)
+            worst_cir = max(worst_cir, z if abs(dmean[i]) > 1e-12 else 0.0)
+            assert abs(dmean[i]) <= 5.0 * se_d[i] + 1e-5 * (abs(mean64) + sd), (i, x, dmean[i], se_d[i])
         else:
             e = abs(mean32 - mean64) / (abs(mean64) + sd)
             worst = max(worst, e)
             assert e <= 1e-5, (i, x, mean32, mean64)
             exact = models.discrete_euler_mean(x)
+            se = sd / np.sqrt(x["paths"])
             assert abs(mean64 - exact) <= 5.0 * se + 1e-12 * abs(exact), (i, x, mean64, exact, se)
-    print("GBM/OU: worst |mean32 - mean64| / (|mean| + sd) = %.3g" % worst)
+    print("GBM/OU: worst |mean32 - mean64| / (|mean| + sd) = %.3g; CIR: worst paired |mean(d)| / se(d) = %.3g"
+          % (worst, worst_cir))
 
 
 def test_c4_full_size_greeks_against_central_differences():

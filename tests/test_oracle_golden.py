@@ -47,6 +47,20 @@ def test_chacha8_zero_key_keystream(oracle):
     assert out.astype("<u4").tobytes() == ks
 
 
+def test_pcg32_step_matches_the_pcg_reference_vectors(oracle):
+    """The seed expansion's PCG32 step (LCG multiplier, XSH-RR output of the advanced state,
+    rotation direction) against the PCG reference's published pcg32 demo output for
+    pcg32_srandom_r(42, 54). srandom sets inc = 2 * 54 + 1, advances from state 0, adds 42 and
+    advances again; each pcg32_random_r then outputs the state before advancing, which is the
+    advance-then-output step (rand_core's order) started one state earlier, at 0 * MUL + inc + 42.
+    The increment seed_from_u64 uses (11634580027462260723) is rand_core's constant and stays
+    unpinned."""
+    inc = (54 << 1) | 1
+    start = (0 * 6364136223846793005 + inc + 42) % 2 ** 64
+    want = [0xa15c02b7, 0x7b47f409, 0xba1d3330, 0x83d2f293, 0xbfa4784b, 0xcbed606e]
+    assert [int(x) for x in oracle.pcg32(start, inc, 6)] == want
+
+
 def test_pseudo_stream_structure(oracle):
     """seed_from_u64 + f64 conversion are restated (parity unpinned); what the maths fixes:
     the f64 draws are multiples of 2^-53 in [0, 1), consecutive blocks use consecutive counters,
