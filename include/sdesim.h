@@ -105,6 +105,21 @@ int sdesim_plan_run(sdesim_plan* plan, void* d_filtration, const double* d_injec
  * out[2q] = mean, out[2q+1] = sample variance (ddof 1). Synchronises the stream. */
 int sdesim_plan_moments(sdesim_plan* plan, double* h_out, void* stream);
 
+/* Allocation-free form of sdesim_plan_run for a plan with SDESIM_OUT_TERMINAL. No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). The kernel's per-block partials
+ * go to a device array of SDESIM_SCRATCH_BYTES inside the library (one per device, kept until the
+ * process ends) instead of the plan's buffer, and the terminal moments, laid out as
+ * sdesim_plan_moments returns them, are written to d_moments (device memory, 2 * (processes + 1
+ * if the plan has a payoff) doubles). The call allocates nothing and copies nothing to the host;
+ * it is asynchronous on `stream`. A plan whose partials (blocks x 2 x quantities x 8 bytes)
+ * exceed SDESIM_SCRATCH_BYTES gets SDESIM_EVALUE (use sdesim_plan_run). Every call on a device
+ * shares that device's array, which keeps the last call's contents: calls that overlap on one
+ * device (other streams, other threads) race, and the caller must order them. */
+#define SDESIM_SCRATCH_BYTES (4u << 20)
+int sdesim_plan_run_scratch(sdesim_plan* plan, void* d_filtration, const double* d_injected, double* d_moments,
+                            void* stream);
+
 /* The reference's long layout (src/filtration.rs:81-116, rows ordered scenario -> time ->
  * process, src/sim/mod.rs:88-91): transpose a [T][P][S] device filtration (f32 or f64) into
  * the value column [S][T][P] as f64 on the device. */

@@ -4,7 +4,19 @@
 //   * sdesim_to_long: [T*P][S] step-major filtration (f32 or f64) -> [S][T*P] f64, the value
 //     column of the reference's long layout (src/filtration.rs:81-116), through LDS tiles so that
 //     both the read and the write are coalesced.
+//   * scratch: the device array behind the allocation-free terminal run (sdesim_plan_run_scratch).
 #include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+
+#include "../../include/sdesim.h"
+
+// Per-block terminal partials of sdesim_plan_run_scratch, which may not allocate: a fixed array in
+// the library's code object, SDESIM_SCRATCH_BYTES per device (each device that runs the library
+// holds its own copy until the process ends). Host code never passes `scratch` itself to a kernel:
+// the name is a host-side stand-in; sdesim_scratch_address gives this device's copy.
+static __device__ __align__(256) double scratch[SDESIM_SCRATCH_BYTES / sizeof(double)];
 
 namespace {
 
@@ -113,6 +125,27 @@ __global__ void __launch_bounds__(256) sdesim_to_long(const T* __restrict__ in, 
 }
 
 }  // namespace
+
+// This device's address of `scratch` (and its size), looked up once per device with that device
+// current and kept.
+hipError_t sdesim_scratch_address(double** ptr, size_t* bytes) {
+  static std::mutex mu;
+  static std::map<int, double*> addr;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(dev);
+  if (it == addr.end()) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    it = addr.emplace(dev, (double*)p).first;
+  }
+  *ptr = it->second;
+  *bytes = SDESIM_SCRATCH_BYTES;
+  return hipSuccess;
+}
 
 hipError_t sdesim_launch_finalize(const double* partials, int nblocks, int kq, long long S, const double* shifts,
                                   double* out, hipStream_t stream) {

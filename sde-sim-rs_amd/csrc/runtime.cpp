@@ -44,6 +44,7 @@ hipError_t sdesim_launch_finalize(const double* partials, int nblocks, int kq, l
                                   double* out, hipStream_t stream);
 hipError_t sdesim_launch_to_long(const void* filtration, int f32, long long S, int rows, double* out,
                                  hipStream_t stream);
+hipError_t sdesim_scratch_address(double** ptr, size_t* bytes);
 
 struct sdesim_universe {
   sdesim::Universe u;
@@ -621,27 +622,53 @@ int64_t sdesim_plan_filtration_bytes(const sdesim_plan* plan) {
 
 const char* sdesim_plan_source(const sdesim_plan* plan) { return plan ? plan->source.c_str() : nullptr; }
 
+// One run of a plan: the fused kernel with its per-block partials in `partials`, then (terminal
+// output) the reduction into `moments`. Checks the buffers the plan's outputs need first.
+static void launch_plan(sdesim_plan* plan, void* d_filtration, const double* d_injected, double* partials,
+                        double* moments, void* stream) {
+  SimArgs a = plan->args;
+  if ((plan->opt.outputs & SDESIM_OUT_FILTRATION) && !d_filtration)
+    fail(SDESIM_EVALUE, "filtration output requested but no buffer given");
+  a.filtration = d_filtration;
+  if (plan->opt.rng == SDESIM_INJECTED && plan->M > 0) {
+    if (!d_injected) fail(SDESIM_EVALUE, "injected uniforms requested but no buffer given");
+    a.injected = d_injected;
+  }
+  a.partials = partials;
+  void* params[] = {&a};
+  hip_check(hipModuleLaunchKernel(plan->kernel.function, (unsigned)plan->grid, 1, 1, (unsigned)plan->block, 1, 1, 0,
+                                  (hipStream_t)stream, params, nullptr),
+            "hipModuleLaunchKernel");
+  if (plan->opt.outputs & SDESIM_OUT_TERMINAL) {
+    hip_check(sdesim_launch_finalize(partials, (int)plan->grid, plan->KQ, a.S, plan->shifts, moments,
+                                     (hipStream_t)stream),
+              "finalize");
+  }
+}
+
 int sdesim_plan_run(sdesim_plan* plan, void* d_filtration, const double* d_injected, void* stream) {
   return guarded([&] {
     if (!plan) fail(SDESIM_EVALUE, "null plan");
     if (plan->P == 0) return;
-    SimArgs a = plan->args;
-    if ((plan->opt.outputs & SDESIM_OUT_FILTRATION) && !d_filtration && plan->P > 0)
-      fail(SDESIM_EVALUE, "filtration output requested but no buffer given");
-    a.filtration = d_filtration;
-    if (plan->opt.rng == SDESIM_INJECTED && plan->M > 0) {
-      if (!d_injected) fail(SDESIM_EVALUE, "injected uniforms requested but no buffer given");
-      a.injected = d_injected;
-    }
-    void* params[] = {&a};
-    hip_check(hipModuleLaunchKernel(plan->kernel.function, (unsigned)plan->grid, 1, 1, (unsigned)plan->block, 1, 1, 0,
-                                    (hipStream_t)stream, params, nullptr),
-              "hipModuleLaunchKernel");
-    if (plan->opt.outputs & SDESIM_OUT_TERMINAL) {
-      hip_check(sdesim_launch_finalize(plan->partials, (int)plan->grid, plan->KQ, a.S, plan->shifts, plan->moments,
-                                       (hipStream_t)stream),
-                "finalize");
-    }
+    launch_plan(plan, d_filtration, d_injected, plan->partials, plan->moments, stream);
+  });
+}
+
+int sdesim_plan_run_scratch(sdesim_plan* plan, void* d_filtration, const double* d_injected, double* d_moments,
+                            void* stream) {
+  return guarded([&] {
+    if (!plan) fail(SDESIM_EVALUE, "null plan");
+    if (!(plan->opt.outputs & SDESIM_OUT_TERMINAL)) fail(SDESIM_EVALUE, "plan has no terminal output");
+    if (!d_moments) fail(SDESIM_EVALUE, "null moments buffer");
+    const unsigned long long need = (unsigned long long)plan->grid * 2ull * (unsigned long long)plan->KQ * 8ull;
+    if (need > SDESIM_SCRATCH_BYTES)
+      fail(SDESIM_EVALUE, "the plan's terminal partials (" + std::to_string(need) + " bytes) exceed the scratch array (" +
+                              std::to_string((unsigned long long)SDESIM_SCRATCH_BYTES) + " bytes); use sdesim_plan_run");
+    double* scr = nullptr;
+    size_t cap = 0;
+    hip_check(sdesim_scratch_address(&scr, &cap), "hipGetSymbolAddress(scratch)");
+    if (need > cap) fail(SDESIM_EVALUE, "scratch array smaller than declared");
+    launch_plan(plan, d_filtration, d_injected, scr, d_moments, stream);
   });
 }
 

@@ -17,6 +17,7 @@ F64, F32 = 0, 1
 COMPAT_REFERENCE, COMPAT_TEXTBOOK = 0, 1
 OUT_FILTRATION, OUT_TERMINAL = 1, 2
 STORAGE_AUTO, STORAGE_TAPE, STORAGE_CHECKPOINTS = 0, 1, 2
+SCRATCH_BYTES = 4 << 20  # SDESIM_SCRATCH_BYTES
 
 
 class Options(ctypes.Structure):
@@ -98,6 +99,8 @@ SIGNATURES = {
     "sdesim_plan_filtration_bytes": (ctypes.c_int64, [ctypes.c_void_p]),
     "sdesim_plan_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "sdesim_plan_moments": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
+    "sdesim_plan_run_scratch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                               ctypes.c_void_p]),
     "sdesim_filtration_to_long": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,
                                                  ctypes.c_int32, ctypes.c_void_p, ctypes.c_void_p]),
     "sdesim_simulate_long": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_int32,
