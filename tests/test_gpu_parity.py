@@ -335,7 +335,8 @@ def test_plan_run_captured_in_a_graph():
     """Plan.run issues launches only (no allocation, memset or synchronisation), so a caller can
     capture it into a graph: the config-1 shape (10,000 paths x 252 steps, launch-bound) captured
     with torch.cuda.graph, replayed three times with eager work and synchronisation in between,
-    gives the eager filtration and moments on every replay."""
+    gives the eager filtration and moments on every replay. The captured launch holds pointers to
+    the plan's device buffers, so `plan` stays referenced for as long as the graph `g` is used."""
     import sde_sim_rs
     times = np.linspace(0.0, 1.0, 253)
     print("case: graph capture of Plan.run (filtration + terminal)", flush=True)
