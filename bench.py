@@ -33,6 +33,8 @@ HESTON = [
 INIT = {"S": 100.0, "v": 0.04}
 PAYOFF = "max(S - 100, 0)"
 HBM_PEAK_GBS = 8000.0
+PROFILE_FILT = "sdesim_rk_f32_sobolxor_filt_heston_2p20x1024.md"
+PROFILE_TERM = "sdesim_rk_f32_sobolxor_term_heston_2p20x1024.md"
 
 
 def _time_plan(plan, out, steps, warmup):
@@ -85,30 +87,33 @@ def _time_fill(buf, steps, warmup):
     return a.elapsed_time(b) / steps
 
 
-def _profile_value(name, key):
-    """A `key: value` line of a committed rocprofv3 summary under profiles/, or None."""
+def _profile_lines(name):
+    """`key: value` lines of a committed rocprofv3 summary under profiles/ (empty if absent)."""
+    out = {}
     try:
         for ln in open(os.path.join(ROOT, "profiles", name)):
-            if ln.startswith(key + ":"):
-                return float(ln.split(":", 1)[1])
+            if ":" in ln and not ln.startswith(("|", "#", " ", "`")):
+                k, v = ln.split(":", 1)
+                out[k.strip()] = v.strip()
     except OSError:
         pass
-    return None
+    return out
 
 
-def _profiled_traffic(paths, nsteps):
-    """HBM bytes per launch of the headline kernel from the committed rocprofv3 counter summary
-    (FETCH_SIZE and WRITE_SIZE runs scaled by a 1 GiB copy); None for other sizes."""
-    if (paths, nsteps) != (1 << 20, 1024):
-        return None
-    path = os.path.join(ROOT, "profiles", "sdesim_rk_f32_sobolxor_filt_heston_2p20x1024.md")
+def _profiled(name, code_hash, keys):
+    """The values of `keys` from a committed counter summary, if it records the counters of this
+    very kernel (its `kernel_code_hash` line equals the plan's code hash): ({key: float}, None),
+    or (None, reason)."""
+    d = _profile_lines(name)
+    if not d:
+        return None, "no counter summary profiles/%s" % name
+    if d.get("kernel_code_hash") != code_hash:
+        return None, ("profiles/%s measured kernel %s; this run's kernel is %s"
+                      % (name, d.get("kernel_code_hash", "(unrecorded)"), code_hash))
     try:
-        for ln in open(path):
-            if ln.startswith("traffic_bytes_per_launch:"):
-                return float(ln.split(":", 1)[1])
-    except OSError:
-        pass
-    return None
+        return {k: float(d[k]) for k in keys}, None
+    except (KeyError, ValueError):
+        return None, "profiles/%s lacks %s" % (name, ", ".join(keys))
 
 
 def host_cpu():
@@ -187,6 +192,7 @@ def main():
         del src, dst
     if "f32" in legs:
         p32 = sde_sim_rs.Plan(uni, times, S, INIT, "sobol", "runge-kutta", precision="f32", **common)
+        res["f32_hash"] = p32.code_hash
         f32 = p32.new_filtration()
         res["f32"] = _time_plan(p32, f32, args.steps, args.warmup)
         res["fill_ms"] = _time_fill(f32, args.steps, args.warmup)
@@ -222,7 +228,7 @@ def main():
             w, k = _time_plan(pt, None, args.steps, args.warmup)
             m = pt.moments()
             term[prec] = {"ms": k, "path_steps_per_s": S * N / (k * 1e-3), "payoff_mean": m["payoff"][0],
-                          "payoff_var": m["payoff"][1], "S_mean": m["S"][0]}
+                          "payoff_var": m["payoff"][1], "S_mean": m["S"][0], "code_hash": pt.code_hash}
         torch.cuda.synchronize()
 
     batch = None
@@ -323,10 +329,16 @@ def main():
         bytes_alg = (N + 1) * 2 * S * 4
         line.update({"value": path_steps / (wall * 1e-3), "ms_per_step": wall})
         ach = bytes_alg / (kern * 1e-3) / 1e9
+        prof, why = (None, "counters are summarised for the 2^20 x 1024 workload only")
+        if (S, N) == (1 << 20, 1024):
+            prof, why = _profiled(PROFILE_FILT, res["f32_hash"], ["traffic_bytes_per_launch"])
         line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": ach / HBM_PEAK_GBS, "traffic": _profiled_traffic(S, N),
-                            "kernel": "sdesim_rk_f32_sobolxor_filt", "kernel_ms": kern,
-                            "algorithmic_bytes": bytes_alg}
+                            "frac": ach / HBM_PEAK_GBS,
+                            "traffic": prof["traffic_bytes_per_launch"] if prof else None,
+                            "kernel": "sdesim_rk_f32_sobolxor_filt", "kernel_code_hash": res["f32_hash"],
+                            "kernel_ms": kern, "algorithmic_bytes": bytes_alg}
+        if why:
+            line["roofline"]["traffic_note"] = why
         fill_gbs = bytes_alg / (res["fill_ms"] * 1e-3) / 1e9
         line["roofline"].update({"fill_same_bytes_gbs": fill_gbs, "frac_of_fill": ach / fill_gbs})
     if "f64" in res:
@@ -344,17 +356,23 @@ def main():
         # from the committed counter summary, over the kernel's time here, against the MI355X peak
         # of one wave64 FP32 VALU instruction per 2 clocks per SIMD (1024 SIMDs at 2.4 GHz);
         # transcendentals run at a quarter of that rate and count 4 slots in the weighted form.
-        prof = "sdesim_rk_f32_sobolxor_term_heston_2p20x1024.md"
-        vi = _profile_value(prof, "valu_instructions_per_path_step")
-        tr = _profile_value(prof, "trans_instructions_per_path_step")
-        if vi is not None and tr is not None and (S, N) == (1 << 20, 1024):
+        prof, why = (None, "counters are summarised for the 2^20 x 1024 workload only")
+        if (S, N) == (1 << 20, 1024):
+            prof, why = _profiled(PROFILE_TERM, term["f32"]["code_hash"],
+                                  ["valu_instructions_per_path_step", "trans_instructions_per_path_step"])
+        if why:
+            term["valu_roofline"] = None
+            term["valu_roofline_note"] = why
+        else:
+            vi, tr = prof["valu_instructions_per_path_step"], prof["trans_instructions_per_path_step"]
             peak = 1024 * 2.4e9 / 2
             rate = vi * S * N / 64 / (term["f32"]["ms"] * 1e-3)
             term["valu_roofline"] = {"bound": "valu", "achieved": rate, "peak": peak,
                                      "unit": "wave64 VALU instructions/s", "frac": rate / peak,
                                      "frac_trans_weighted": rate * (vi + 3 * tr) / vi / peak,
                                      "valu_per_path_step": vi, "trans_per_path_step": tr,
-                                     "kernel": "sdesim_rk_f32_sobolxor_term"}
+                                     "kernel": "sdesim_rk_f32_sobolxor_term",
+                                     "kernel_code_hash": term["f32"]["code_hash"]}
         line["terminal_only"] = term
     if "long" in res:
         line["long_layout"] = res["long"]

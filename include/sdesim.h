@@ -135,6 +135,11 @@ int sdesim_simulate_long(const sdesim_universe* u, const double* times, int32_t 
 
 /* Generated HIP source of a plan's kernel (for inspection); valid while the plan lives. */
 const char* sdesim_plan_source(const sdesim_plan* plan);
+/* The identity of the plan's compiled kernel: a 64-bit FNV-1a hash of its generated source, the
+ * device headers, the compile options and the hipRTC version (the key of the on-disk kernel
+ * cache), so measurements of a kernel can be tied to the code they measured. 0 for a plan with
+ * no kernel. No reference counterpart. */
+uint64_t sdesim_plan_code_hash(const sdesim_plan* plan);
 
 /* Host-only helpers (no device needed): the kernel source a plan would generate for a time grid
  * of n_times points, and a gfx950 compile of a source through hipRTC. `needed` receives the

@@ -148,12 +148,9 @@ uint64_t fnv1a(const void* data, size_t n, uint64_t h = 0xcbf29ce484222325ull) {
 
 constexpr uint64_t kCodeMagic = 0x314b4f43454453ull;  // "SDECOK1"
 
-std::string code_cache_file(const std::string& src) {
-  const char* dir = std::getenv("SDESIM_CACHE_DIR");
-  std::string d;
-  if (dir && *dir) d = dir;
-  else if (const char* home = std::getenv("HOME")) d = std::string(home) + "/.cache/sdesim";
-  else return std::string();
+// The identity of a run-time kernel's code object: FNV-1a over the generated source, both device
+// headers, the compile options and the hipRTC version.
+uint64_t code_key(const std::string& src) {
   int major = 0, minor = 0;
   hiprtcVersion(&major, &minor);
   uint64_t h = fnv1a(src.data(), src.size());
@@ -162,9 +159,17 @@ std::string code_cache_file(const std::string& src) {
   h = fnv1a(sdesim_embedded_args_header, std::strlen(sdesim_embedded_args_header), h);
   for (const char* o : kRtcOptions) h = fnv1a(o, std::strlen(o) + 1, h);
   const int ver[2] = {major, minor};
-  h = fnv1a(ver, sizeof ver, h);
+  return fnv1a(ver, sizeof ver, h);
+}
+
+std::string code_cache_file(const std::string& src) {
+  const char* dir = std::getenv("SDESIM_CACHE_DIR");
+  std::string d;
+  if (dir && *dir) d = dir;
+  else if (const char* home = std::getenv("HOME")) d = std::string(home) + "/.cache/sdesim";
+  else return std::string();
   char name[40];
-  std::snprintf(name, sizeof name, "/k_%016llx.co", (unsigned long long)h);
+  std::snprintf(name, sizeof name, "/k_%016llx.co", (unsigned long long)code_key(src));
   return d + name;
 }
 
@@ -621,6 +626,15 @@ int64_t sdesim_plan_filtration_bytes(const sdesim_plan* plan) {
 }
 
 const char* sdesim_plan_source(const sdesim_plan* plan) { return plan ? plan->source.c_str() : nullptr; }
+
+uint64_t sdesim_plan_code_hash(const sdesim_plan* plan) {
+  if (!plan || plan->source.empty()) return 0;
+  try {
+    return code_key(plan->source);
+  } catch (...) {
+    return 0;
+  }
+}
 
 // One run of a plan: the fused kernel with its per-block partials in `partials`, then (terminal
 // output) the reduction into `moments`. Checks the buffers the plan's outputs need first.

@@ -99,6 +99,7 @@ SIGNATURES = {
     "sdesim_plan_filtration_bytes": (ctypes.c_int64, [ctypes.c_void_p]),
     "sdesim_plan_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "sdesim_plan_moments": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p]),
+    "sdesim_plan_code_hash": (ctypes.c_uint64, [ctypes.c_void_p]),
     "sdesim_plan_run_scratch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_void_p]),
     "sdesim_filtration_to_long": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_int32,

@@ -183,6 +183,13 @@ class Plan:
         s = nat.lib.sdesim_plan_source(self._h)
         return s.decode("utf-8") if s else ""
 
+    @property
+    def code_hash(self):
+        """Identity of the compiled kernel (source, device headers, options, hipRTC version), as
+        16 hex digits; '' for a plan with no kernel."""
+        h = nat.lib.sdesim_plan_code_hash(self._h)
+        return "%016x" % h if h else ""
+
     def new_filtration(self, device="cuda"):
         import torch
         return torch.empty(self.shape, dtype=self.dtype, device=device)

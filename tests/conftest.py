@@ -5,7 +5,10 @@ import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-for p in (ROOT, os.path.join(ROOT, "sde-sim-rs_amd")):
+# SDESIM_TEST_PACKAGE_ROOT: the directory holding the sde_sim_rs package under test (the sanitized
+# build's copy, tests/test_sanitized_host.py); the in-tree package otherwise.
+_PKG_ROOT = os.environ.get("SDESIM_TEST_PACKAGE_ROOT") or os.path.join(ROOT, "sde-sim-rs_amd")
+for p in (ROOT, _PKG_ROOT):
     if p not in sys.path:
         sys.path.insert(0, p)
 
