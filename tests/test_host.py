@@ -519,3 +519,21 @@ def test_plan_refuses_initial_values_of_the_wrong_length():
     for bad in ([100.0], [100.0, 0.04, 1.0], [[100.0, 0.04]], np.zeros((2, 1))):
         with pytest.raises(ValueError, match="one per process"):
             sde_sim_rs.Plan(u, times, 8, bad, "sobol", "runge-kutta")
+
+
+def test_bench_quotes_counters_only_for_the_kernel_they_measured():
+    """bench.py takes traffic and VALU figures from the committed summaries only when the summary
+    records the running kernel's code hash; otherwise it reports none, with a reason."""
+    import importlib
+    import sys
+    sys.path.insert(0, ROOT)
+    bench = importlib.import_module("bench")
+    for name, keys in ((bench.PROFILE_FILT, ["traffic_bytes_per_launch"]),
+                       (bench.PROFILE_TERM, ["valu_instructions_per_path_step", "trans_instructions_per_path_step"])):
+        recorded = bench._profile_lines(name).get("kernel_code_hash")
+        assert recorded and re.fullmatch(r"[0-9a-f]{16}", recorded), name
+        vals, why = bench._profiled(name, recorded, keys)
+        assert why is None and all(vals[k] > 0 for k in keys)
+        vals, why = bench._profiled(name, "0" * 16, keys)
+        assert vals is None and recorded in why and "0" * 16 in why
+    assert bench._profiled("no_such_summary.md", "0" * 16, ["x"])[0] is None
