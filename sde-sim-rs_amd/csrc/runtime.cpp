@@ -11,8 +11,10 @@
 
 #include <unistd.h>
 
+#include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -49,6 +51,8 @@ hipError_t sdesim_scratch_address(double** ptr, size_t* bytes);
 struct sdesim_universe {
   sdesim::Universe u;
 };
+
+void sdesim_log(const char* fmt, ...);
 
 namespace {
 
@@ -202,10 +206,14 @@ CompiledKernel compile_kernel(const std::string& src, const std::string& name) {
   std::lock_guard<std::mutex> lock(g_cache_mu);
   auto key = std::make_pair(dev, src);
   auto it = g_kernels.find(key);
-  if (it != g_kernels.end()) return it->second;
+  if (it != g_kernels.end()) {
+    sdesim_log("kernel %s: already loaded on device %d", name.c_str(), dev);
+    return it->second;
+  }
   const std::string path = code_cache_file(src);
   CompiledKernel k;
   bool loaded = false;
+  const auto t0 = std::chrono::steady_clock::now();
   if (!path.empty()) {
     const std::vector<char> code = read_cached_code(path);
     if (!code.empty()) {
@@ -219,6 +227,9 @@ CompiledKernel compile_kernel(const std::string& src, const std::string& name) {
     if (!path.empty()) write_cached_code(path, code);
   }
   hip_check(hipModuleGetFunction(&k.function, k.module, name.c_str()), "hipModuleGetFunction");
+  sdesim_log("kernel %s: %s in %.1f ms on device %d", name.c_str(),
+             loaded ? ("loaded from " + path).c_str() : "compiled with hipRTC",
+             std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count(), dev);
   g_kernels[key] = k;
   return k;
 }
@@ -347,6 +358,27 @@ T* upload(const std::vector<T>& v) {
 }
 
 }  // namespace
+
+// Host-side log lines on stderr, one per event (plan creation, kernel compile or cache load, batch
+// autotuning, adjoint storage choice), when SDESIM_LOG is set to anything but "" or "0". Logging
+// changes no result.
+bool sdesim_log_enabled() {
+  static const bool on = [] {
+    const char* v = std::getenv("SDESIM_LOG");
+    return v && *v && std::strcmp(v, "0") != 0;
+  }();
+  return on;
+}
+
+void sdesim_log(const char* fmt, ...) {
+  if (!sdesim_log_enabled()) return;
+  char line[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(line, sizeof line, fmt, ap);
+  va_end(ap);
+  std::fprintf(stderr, "[sdesim] %s\n", line);
+}
 
 int sdesim_set_error(int code, const std::string& msg) {
   g_last_error = msg;
@@ -613,6 +645,8 @@ int sdesim_plan_create(const sdesim_universe* uh, const double* times, int32_t T
       plan->shifts = plan->own(upload(shifts));
       a.partials = plan->partials;
     }
+    sdesim_log("plan %s: %lld paths, %d steps, %d processes, %d increments, grid %lld x %d, %d path(s) per lane",
+               co.name.c_str(), (long long)S, N, P, M, plan->grid, plan->block, plan->ppl);
     *out = plan.release();
   });
 }
@@ -798,8 +832,11 @@ void sdesim_kernel_cache_reset(void) {
 
 int sdesim_compile_source(const char* source, int64_t* code_bytes) {
   return guarded([&] {
+    const auto t0 = std::chrono::steady_clock::now();
     std::vector<char> code = rtc_compile(source ? source : "");
     if (code_bytes) *code_bytes = (int64_t)code.size();
+    sdesim_log("source compiled with hipRTC: %zu bytes of code in %.1f ms", code.size(),
+               std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count());
   });
 }
 

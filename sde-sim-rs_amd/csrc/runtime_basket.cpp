@@ -16,6 +16,7 @@
 hipError_t sdesim_launch_basket(int f32, int assets, int mode, int grid, const BasketArgs& a, int nq,
                                 double* sums, hipStream_t stream);
 int sdesim_set_error(int code, const std::string& msg);
+void sdesim_log(const char* fmt, ...);
 
 namespace {
 constexpr int kSeg = 19;  // checkpoint interval of the compiled reverse sweep
@@ -98,6 +99,8 @@ int sdesim_basket_create(const sdesim_basket_spec* spec, sdesim_basket** out) {
     }
     bk(hipMalloc(&b->partials, sizeof(double) * b->nq * b->grid), "hipMalloc(partials)");
     bk(hipMalloc(&b->sums, sizeof(double) * b->nq), "hipMalloc(sums)");
+    sdesim_log("basket: %d assets, %lld paths, %d steps, adjoint storage %s", s.assets, (long long)s.paths, s.steps,
+               b->storage == SDESIM_STORAGE_TAPE ? "tape" : "checkpoints");
     *out = b.release();
   });
 }

@@ -35,6 +35,7 @@ hipError_t sdesim_batch_occupancy(int f32, int ppl, int block, int* blocks_per_c
 
 // Error plumbing shared with runtime.cpp.
 int sdesim_set_error(int code, const std::string& msg);
+void sdesim_log(const char* fmt, ...);
 
 namespace {
 
@@ -303,6 +304,8 @@ int sdesim_batch_autotune(sdesim_batch* b, void* stream, int32_t* ppl_out, int32
       tune_store(b->key, best_p, best_b, best);
     }
     b->configure(best_p, best_b);
+    sdesim_log("batch autotune: %d path(s) per lane, block %d (%s, %.3f ms)", best_p, best_b,
+               best < 0 ? "remembered" : "timed", best);
     if (ppl_out) *ppl_out = best_p;
     if (block_out) *block_out = best_b;
     if (ms_out) *ms_out = best;

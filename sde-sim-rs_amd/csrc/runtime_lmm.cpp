@@ -14,6 +14,7 @@
 hipError_t sdesim_launch_lmm(int f32, int mode, int grid, const LmmArgs& a, hipStream_t stream);
 hipError_t sdesim_launch_sum_partials(const double* partials, int nblocks, int nq, double* out, hipStream_t stream);
 int sdesim_set_error(int code, const std::string& msg);
+void sdesim_log(const char* fmt, ...);
 void sdesim_host_sobol_tables(int dims, int nslices, std::vector<unsigned int>& out);
 void sdesim_chacha8_words(unsigned long long seed, size_t n, std::vector<unsigned int>& out);
 extern "C" const int sdesim_jk_dims;  // dimensions of the embedded Joe-Kuo table
@@ -129,6 +130,8 @@ int sdesim_lmm_create(const sdesim_lmm_spec* spec, sdesim_lmm** out) {
     lk(hipMalloc(&m->parf, sizeof(float) * kNF * (1 + kF)), "hipMalloc(parameters)");
     lk(hipMalloc(&m->partials, sizeof(double) * kNQ * m->grid), "hipMalloc(partials)");
     lk(hipMalloc(&m->sums, sizeof(double) * kNQ), "hipMalloc(sums)");
+    sdesim_log("forward-rate model: %lld paths, %d steps, adjoint storage %s", (long long)s.paths, s.steps,
+               m->storage == SDESIM_STORAGE_TAPE ? "tape" : "checkpoints");
     *out = m.release();
   });
 }

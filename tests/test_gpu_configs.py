@@ -275,3 +275,42 @@ def test_c5_full_size_sensitivities_against_central_differences():
         worst = max(worst, err)
         assert err <= 1e-4, (i, adj["dL0"][i], fd)
     print("max relative error of 40 curve sensitivities vs FD at 2^20 x 160: %.3g" % worst)
+
+
+def test_benchmarked_workloads_rerun_bitwise():
+    """Run again, every benchmarked workload gives bitwise the same result: the timed filtration
+    (2^20 x 1024, FP32), the 4096-instance batch (its waves claim work items from an atomic
+    counter, so which wave runs which item changes between runs, while the partial sums are
+    placed by item), and the basket and forward-rate adjoints at their full sizes."""
+    import sde_sim_rs
+    from sde_sim_rs import models
+    from sde_sim_rs.adjoint import Basket, basket_config
+    from sde_sim_rs.lmm import ForwardRateModel, forward_rate_config
+    times = np.linspace(0.0, 1.0, FULL_N + 1)
+    print("case: rerun bitwise: filtration, batch, basket, forward-rate", flush=True)
+    p = sde_sim_rs.Plan(HESTON, times, FULL_S, INIT, "sobol", "runge-kutta", precision="f32", **BENCH_OPTS)
+    f1, f2 = p.new_filtration(), p.new_filtration()
+    p.run(f1)
+    p.run(f2)
+    torch.cuda.synchronize()
+    assert torch.equal(f1, f2)
+    del f1, f2
+    torch.cuda.empty_cache()
+    b = sde_sim_rs.Batch(models.batched_instances(n=4096, seed=11), precision="f32")
+    b.run()
+    m1 = b.moments()
+    b.run()
+    assert np.array_equal(m1, b.moments())
+    del b
+    c = basket_config(seed=3)
+    bk = Basket(8, 365, 1 << 22, horizon=1.0, strike=100.0, width=0.5, seed=3, precision="f32")
+    r1 = bk.evaluate(c["x0"], c["mu"], c["sigma"], c["rho"], adjoint=True)
+    r2 = bk.evaluate(c["x0"], c["mu"], c["sigma"], c["rho"], adjoint=True)
+    for k in ("price", "x0", "mu", "sigma", "rho"):
+        assert np.array_equal(r1[k], r2[k]), k
+    del bk
+    c = forward_rate_config(seed=5)
+    fm = ForwardRateModel(steps=160, paths=1 << 20, expiry_index=4, strike=0.03, scramble_seed=5, precision="f32")
+    r1 = fm.evaluate(c["L0"], c["vol"], c["loadings"])
+    r2 = fm.evaluate(c["L0"], c["vol"], c["loadings"])
+    assert r1["price"] == r2["price"] and np.array_equal(r1["dL0"], r2["dL0"])

@@ -537,3 +537,22 @@ def test_bench_quotes_counters_only_for_the_kernel_they_measured():
         vals, why = bench._profiled(name, "0" * 16, keys)
         assert vals is None and recorded in why and "0" * 16 in why
     assert bench._profiled("no_such_summary.md", "0" * 16, ["x"])[0] is None
+
+
+def test_log_lines_only_when_asked():
+    """SDESIM_LOG=1 writes one host-side line per event to stderr (here: a hipRTC compile); without
+    it the library prints nothing."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r); import ctypes; from sde_sim_rs import _native as nat; "
+            "n = ctypes.c_int64(); rc = nat.lib.sdesim_compile_source("
+            "b'extern \"C\" __global__ void k(float* x) { x[0] = 1.0f; }', ctypes.byref(n)); "
+            "print(rc, n.value > 0)" % os.path.join(ROOT, "sde-sim-rs_amd"))
+    for value, expect in (("1", True), ("0", False), (None, False)):
+        env = dict(os.environ)
+        env.pop("SDESIM_LOG", None)
+        if value is not None:
+            env["SDESIM_LOG"] = value
+        r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+        assert r.stdout.strip() == "0 True", (r.stdout, r.stderr[-2000:])
+        assert ("[sdesim] source compiled with hipRTC" in r.stderr) == expect, r.stderr[-2000:]
