@@ -90,6 +90,9 @@ int32_t sdesim_process_index(const sdesim_universe* u, const char* name); /* reg
  * expression over process names and t evaluated on the terminal row. */
 int sdesim_plan_create(const sdesim_universe* u, const double* times, int32_t n_times, const double* x0,
                        int64_t scenarios, const sdesim_options* opt, const char* payoff, sdesim_plan** out);
+/* Frees the plan. A plan whose run was issued on a capturing stream (HIP graph capture) keeps its
+ * device buffers (time grid, initial row, tables, partials) until the process ends, since the
+ * captured graph may still launch with them. */
 void sdesim_plan_free(sdesim_plan* plan);
 
 /* Bytes of the [T][P][S] filtration buffer the plan writes (4 or 8 bytes per value). */

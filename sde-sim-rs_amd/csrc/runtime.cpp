@@ -434,9 +434,18 @@ struct sdesim_plan {
   double* partials = nullptr;
   double* moments = nullptr;
   double* shifts = nullptr;
+  // Set when a run was issued on a capturing stream: the captured graph holds pointers to the
+  // device buffers, and nothing signals the graph's destruction, so they are kept until exit.
+  bool captured = false;
   ~sdesim_plan() {
+    if (captured) {
+      sdesim_log("plan %s freed after a graph capture of its run: its %zu device buffers stay allocated", kname.c_str(),
+                 owned.size());
+      return;
+    }
     for (void* p : owned) (void)hipFree(p);
   }
+  std::string kname;
   template <class T>
   T* own(T* p) {
     if (p) owned.push_back((void*)p);
@@ -572,6 +581,7 @@ int sdesim_plan_create(const sdesim_universe* uh, const double* times, int32_t T
     co.uniform_grid = ug;
     co.name = sdesim::kernel_name(co);
     plan->source = sdesim::generate_kernel_source(U, co);
+    plan->kname = co.name;
     plan->kernel = compile_kernel(plan->source, co.name);
     std::vector<double> x0v(x0 ? x0 : nullptr, x0 ? x0 + P : nullptr);
     if (!x0) x0v.assign(P, 0.0);
@@ -674,6 +684,11 @@ uint64_t sdesim_plan_code_hash(const sdesim_plan* plan) {
 // output) the reduction into `moments`. Checks the buffers the plan's outputs need first.
 static void launch_plan(sdesim_plan* plan, void* d_filtration, const double* d_injected, double* partials,
                         double* moments, void* stream) {
+  if (stream) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing((hipStream_t)stream, &cs) == hipSuccess && cs == hipStreamCaptureStatusActive)
+      plan->captured = true;
+  }
   SimArgs a = plan->args;
   if ((plan->opt.outputs & SDESIM_OUT_FILTRATION) && !d_filtration)
     fail(SDESIM_EVALUE, "filtration output requested but no buffer given");

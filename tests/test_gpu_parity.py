@@ -5,10 +5,12 @@ differences are last-ulp libm results and FMA contraction on the device). The ps
 stream is the same ChaCha8 word stream on both sides, so "identical uniforms" holds for the
 pseudo, Sobol and injected modes alike.
 """
+import os
+
 import numpy as np
 import pytest
 
-from conftest import GBM, HESTON, JUMP, JUMP_X0
+from conftest import GBM, HESTON, JUMP, JUMP_X0, ROOT
 
 pytestmark = pytest.mark.gpu
 
@@ -336,7 +338,8 @@ def test_plan_run_captured_in_a_graph():
     capture it into a graph: the config-1 shape (10,000 paths x 252 steps, launch-bound) captured
     with torch.cuda.graph, replayed three times with eager work and synchronisation in between,
     gives the eager filtration and moments on every replay. The captured launch holds pointers to
-    the plan's device buffers, so `plan` stays referenced for as long as the graph `g` is used."""
+    the plan's device buffers; a plan whose run was captured keeps them until the process ends
+    (test_captured_plan_keeps_its_buffers)."""
     import sde_sim_rs
     times = np.linspace(0.0, 1.0, 253)
     print("case: graph capture of Plan.run (filtration + terminal)", flush=True)
@@ -618,3 +621,38 @@ def test_random_factored_universes_fp32(oracle):
         ref = oracle.simulate(eqs, times, 300, init, scheme=scheme, rng=rng_method, seed=seed, scramble=scramble,
                               scramble_seed=seed + 1, compat=compat)
         _close(f.transpose(2, 0, 1), ref, rtol=1e-4, atol=1e-5 * max(1.0, float(np.abs(ref).max())))
+
+
+def test_captured_plan_keeps_its_buffers():
+    """A plan whose run was captured into a graph keeps its device buffers when it is freed (the
+    graph may still launch with them); a plan that was only run eagerly frees them. Checked through
+    the library's log in a separate process; the graph is not replayed after the free."""
+    import subprocess
+    import sys
+    code = """
+import sys, gc
+sys.path[:0] = [%r, %r]
+import numpy as np, torch, sde_sim_rs
+from conftest import GBM
+times = np.linspace(0.0, 1.0, 17)
+eager = sde_sim_rs.Plan(GBM, times, 1000, {"X": 100.0}, "pseudo", "euler", seed=1)
+f = eager.new_filtration(); eager.run(f); torch.cuda.synchronize()
+del eager; gc.collect()
+print("eager freed", flush=True)
+plan = sde_sim_rs.Plan(GBM, times, 1000, {"X": 100.0}, "pseudo", "euler", seed=2)
+f2 = plan.new_filtration()
+plan.run(f2); torch.cuda.synchronize()
+g = torch.cuda.CUDAGraph()
+with torch.cuda.graph(g):
+    plan.run(f2)
+torch.cuda.synchronize()
+del plan; gc.collect()
+print("captured freed", flush=True)
+""" % (os.path.join(ROOT, "sde-sim-rs_amd"), os.path.join(ROOT, "tests"))
+    print("case: captured plan keeps its buffers", flush=True)
+    env = dict(os.environ, SDESIM_LOG="1")
+    r = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr[-3000:]
+    assert "eager freed" in r.stdout and "captured freed" in r.stdout
+    kept = [ln for ln in r.stderr.splitlines() if "freed after a graph capture" in ln]
+    assert len(kept) == 1, r.stderr[-3000:]
