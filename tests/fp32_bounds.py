@@ -22,9 +22,9 @@ state (S, v) and for the probe row that the next step's stage 1 reads:
 * errors propagate through every coefficient, increment and stage to first order in each error
   and with their products (error x error) added, so a larger incoming error cannot hide.
 
-The bound makes no use of any FP32 result. tests/test_host.py checks it against an f32
-evaluation of the step in numpy (which must stay inside it) and against a planted scheme
-mistake (which must leave it).
+The bound makes no use of any FP32 result. tests/test_fp32_bounds.py checks it against an f32
+evaluation of the step in numpy (which must stay inside it) and against planted mistakes (which
+must leave it), for the Sobol-XOR uniforms and for the pseudo-random draws.
 """
 import numpy as np
 
@@ -66,6 +66,32 @@ def psi_error(y):
     return np.where(qi < 1233, U * np.abs(xp), err)
 
 
+def psi_error_words(u):
+    """Bound on |Psi_f32(u) - Psi_f64(u)| (units of z) for the pseudo-random (and FP32 Sobol-shift)
+    form, which takes q = min(u, 1 - u) from the draw's two 32-bit words: q' = fma(f32(hi),
+    2^-32, f32(lo) 2^-64) is within 2u q + 2^-53 of the f64 uniform's q; t = sqrt(-2 ln 2 *
+    log2 q') with the constant rounded to f32; x = t - num(t) / den(t) with the A&S constants
+    rounded to f32. q' < 2.87e-7 (|z| > 5) takes the f64 formula on the exact q, rounded once."""
+    u = np.asarray(u, dtype=np.float64)
+    q = np.where(u < 0.5, u, 1.0 - u)
+    L = -np.log2(q)
+    t = np.sqrt(2.0 * np.log(2.0) * L)
+    num = (0.010328 * t + 0.802853) * t + 2.515517
+    den = ((0.001308 * t + 0.189269) * t + 1.432788) * t + 1.0
+    R = num / den
+    x = t - R
+    dnum = 2.0 * 0.010328 * t + 0.802853
+    dden = (3.0 * 0.001308 * t + 2.0 * 0.189269) * t + 1.432788
+    dxdt = np.abs(1.0 - (dnum * den - num * dden) / (den * den))
+    # q: 2u q + 2^-53 (relative 2.01u for q >= 2.87e-7); L: 4u L + that / ln 2; the argument of
+    # the square root: L's error + 2u (the constant and the product); t: half of it + 4u.
+    dq = 2.0 * U + 2.0 ** -53 / q
+    dL = 4.0 * U * L + dq / np.log(2.0)
+    dt = t * (0.5 * (dL / L + 2.0 * U) + 4.0 * U)
+    err = dxdt * dt + 24.0 * U * R + U * np.abs(x)
+    return np.where(q < 2.87e-7, U * np.abs(x), err)
+
+
 def _kink(a, e):
     """sup over |w - a| <= e of |sqrt(max(w, 0)) - sqrt(max(a, 0))|."""
     ap = np.maximum(a, 0.0)
@@ -79,10 +105,12 @@ def _kink(a, e):
 
 
 def heston_rk_bound(y, times, x0=(100.0, 0.04), r=0.02, kappa=2.0, theta=0.04, c1=-0.7 * 0.5,
-                    c2=0.7141428428542850 * 0.5, states=None, perturb=None):
+                    c2=0.7141428428542850 * 0.5, states=None, perturb=None, draws="xor"):
     """FP64 paths and FP32 error bounds of the Heston RK scheme with the reference's cache.
 
-    y: uint32 [S][N * 2], the XOR-scrambled Sobol words of each path (dimension 2 n + k).
+    y: the uniforms of each path, [S][N * 2] (draw or dimension 2 n + k): with draws="xor" the
+    XOR-scrambled Sobol words (uint32; u = (y + 0.5) 2^-32), with draws="words" the f64 uniforms
+    of the pseudo-random stream, whose FP32 inverse normal is formed from the draw's two words.
     Model: dS = r S dt + sqrt(v+) S dW1, dv = kappa (theta - v+) dt + c1 sqrt(v+) dW1
     + c2 sqrt(v+) dW2 (v+ = max(v, 0)). Returns (values, bound), each [S][T][2] f64.
 
@@ -99,7 +127,7 @@ def heston_rk_bound(y, times, x0=(100.0, 0.04), r=0.02, kappa=2.0, theta=0.04, c
     `perturb` (tests of the bound only) replaces the scheme: "textbook" evaluates stage 1 on the
     settled row instead of the previous probe.
     """
-    y = np.asarray(y, dtype=np.uint32)
+    y = np.asarray(y, dtype=np.uint32 if draws == "xor" else np.float64)
     S = y.shape[0]
     times = np.asarray(times, dtype=np.float64)
     N = len(times) - 1
@@ -131,12 +159,17 @@ def heston_rk_bound(y, times, x0=(100.0, 0.04), r=0.02, kappa=2.0, theta=0.04, c
         dt = times[n + 1] - times[n]
         sdt = np.sqrt(dt)
         y1, y2 = y[:, 2 * n], y[:, 2 * n + 1]
-        u1 = (y1.astype(np.float64) + 0.5) * 2.0 ** -32
-        u2 = (y2.astype(np.float64) + 0.5) * 2.0 ** -32
+        if draws == "xor":
+            u1 = (y1.astype(np.float64) + 0.5) * 2.0 ** -32
+            u2 = (y2.astype(np.float64) + 0.5) * 2.0 ** -32
+            e1, e2 = C_PSI * psi_error(y1), C_PSI * psi_error(y2)
+        else:
+            u1, u2 = y1, y2
+            e1, e2 = psi_error_words(u1), psi_error_words(u2)
         dw1 = sdt * inv_normal(u1)
         dw2 = sdt * inv_normal(u2)
-        edw1 = sdt * C_PSI * psi_error(y1) + 4.0 * U * a(dw1)
-        edw2 = sdt * C_PSI * psi_error(y2) + 6.0 * U * a(dw2)  # (c2 sqrt(dt)) C x' in f32
+        edw1 = sdt * e1 + 4.0 * U * a(dw1)
+        edw2 = sdt * e2 + 6.0 * U * a(dw2)  # (c2 sqrt(dt)) times the increment in f32
         sk = np.where(u1 > 0.5, 1.0, -1.0)
         adt, kdt = r * dt, kappa * dt
         g = c1 * dw1 + c2 * dw2
@@ -181,12 +214,12 @@ def heston_rk_bound(y, times, x0=(100.0, 0.04), r=0.02, kappa=2.0, theta=0.04, c
 
 
 def emulate_f32(y, times, x0=(100.0, 0.04), r=0.02, kappa=2.0, theta=0.04, c1=-0.7 * 0.5,
-                c2=0.7141428428542850 * 0.5):
+                c2=0.7141428428542850 * 0.5, draws="xor"):
     """The same Heston RK recursion evaluated in numpy float32 (each operation rounded; the
     increments from the rescaled rational in t' = sqrt(-log2 q) and the f64-tabulated tail),
     [S][T][2] as float64. An independent FP32 evaluation for checking the bound on the CPU."""
     f = np.float32
-    y = np.asarray(y, dtype=np.uint32)
+    y = np.asarray(y, dtype=np.uint32 if draws == "xor" else np.float64)
     S = y.shape[0]
     times = np.asarray(times, dtype=np.float64)
     N = len(times) - 1
@@ -209,17 +242,36 @@ def emulate_f32(y, times, x0=(100.0, 0.04), r=0.02, kappa=2.0, theta=0.04, c1=-0
             xp[tail] = f(-inv_normal(qq) / C_PSI)
         return xp, yw < np.uint32(1 << 31)  # x' >= 0 and whether u < 1/2
 
+    def zwords(uw):
+        # |z| and whether u < 1/2, from q rounded to f32 (the words form's q' up to 2^-53).
+        q = np.where(uw < 0.5, uw, 1.0 - uw)
+        q32 = q.astype(f)
+        t = np.sqrt(f(1.3862943611198906) * -np.log2(q32))
+        num = (f(0.010328) * t + f(0.802853)) * t + f(2.515517)
+        den = ((f(0.001308) * t + f(0.189269)) * t + f(1.432788)) * t + f(1.0)
+        x = t - num * (f(1.0) / den)
+        tail = q < 2.87e-7
+        if tail.any():
+            x[tail] = f(-inv_normal(q[tail]))
+        return x, uw < 0.5
+
     cs, cv = s.copy(), v.copy()
     for n in range(N):
         dt = f(times[n + 1] - times[n])
         sdt = f(np.sqrt(times[n + 1] - times[n]))
-        xp1, neg1 = xprime(y[:, 2 * n])
-        xp2, neg2 = xprime(y[:, 2 * n + 1])
-        m1 = sdt * f(C_PSI)
-        m2 = (f(c2) * sdt) * f(C_PSI)
+        if draws == "xor":
+            xp1, neg1 = xprime(y[:, 2 * n])
+            xp2, neg2 = xprime(y[:, 2 * n + 1])
+            m1 = sdt * f(C_PSI)
+            m2 = (f(c2) * sdt) * f(C_PSI)
+        else:
+            xp1, neg1 = zwords(y[:, 2 * n])
+            xp2, neg2 = zwords(y[:, 2 * n + 1])
+            m1 = sdt
+            m2 = f(c2) * sdt
         w0 = np.where(neg1, -m1, m1) * xp1
         w1 = np.where(neg2, -m2, m2) * xp2
-        sks = np.where(neg1, -sdt, sdt)
+        sks = np.where(neg1, -sdt, sdt) if draws == "xor" else np.where(y[:, 2 * n] > 0.5, sdt, -sdt)
         g0 = f(c1) * w0 + w1
 
         def co(ps, pv):

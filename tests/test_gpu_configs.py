@@ -86,6 +86,42 @@ def test_c2_fp32_timed_kernel_against_oracle(oracle):
     assert abs(pay_g - pay_r) <= 1e-6 * abs(pay_r), (pay_g, pay_r)
 
 
+def test_c2_model_fp32_pseudo_random_against_oracle(oracle):
+    """The same Heston RK model in FP32 with the reference's default generator, pseudo-random
+    ChaCha8 draws (path s seeded seed + s), whose inverse normal is formed from the draw's two
+    words: 4096 paths x 1024 steps, every value against the oracle's FP64 step applied to the
+    kernel's own row within one FP32 step's rounding (the words form's budget in
+    tests/fp32_bounds.py), the first 64 steps against the propagated bound, and the terminal
+    payoff mean within 1e-6 of the oracle's."""
+    import sde_sim_rs
+    import fp32_bounds as B
+    S, seed = 4096, 42
+    times = np.linspace(0.0, 1.0, FULL_N + 1)
+    print("case: c2 model fp32 pseudo S=%d N=%d" % (S, FULL_N), flush=True)
+    plan = sde_sim_rs.Plan(HESTON, times, S, INIT, "pseudo", "runge-kutta", precision="f32", seed=seed)
+    assert "sdesim_rk_f32_pseudo_filt" in plan.source
+    f = plan.new_filtration()
+    plan.run(f)
+    torch.cuda.synchronize()
+    gpu = f.cpu().numpy().astype(np.float64).transpose(2, 0, 1)
+    ref = oracle.simulate(HESTON, times, S, INIT, scheme="runge-kutta", rng="pseudo", seed=seed)
+    y = np.stack([oracle.pseudo_f64(seed + s, 2 * FULL_N) for s in range(S)])
+    X, E = B.heston_rk_bound(y, times, draws="words")
+    _close(X, ref, 1e-12, 1e-12)
+    n = 65
+    err = np.abs(gpu[:, :n] - ref[:, :n])
+    tol = E[:, :n] + 1e-12 * np.abs(ref[:, :n]) + 1e-13
+    assert np.all(err <= tol), ("first 64 steps", float(np.max(err / tol)))
+    P, R = B.heston_rk_bound(y, times, states=gpu, draws="words")
+    d = np.abs(gpu[:, 1:] - P[:, 1:])
+    r = R[:, 1:] + 1e-12 * np.abs(P[:, 1:]) + 1e-13
+    print("one-step check: worst |gpu - step(gpu)| / bound = %.3g" % float(np.max(d / r)), flush=True)
+    assert np.all(d <= r)
+    pay_g = np.maximum(gpu[:, -1, 0] - 100.0, 0.0).mean()
+    pay_r = np.maximum(ref[:, -1, 0] - 100.0, 0.0).mean()
+    assert abs(pay_g - pay_r) <= 1e-6 * abs(pay_r), (pay_g, pay_r)
+
+
 def test_c2_full_size_terminal_moments():
     """2^20 paths x 1024 steps, the timed configuration: the FP32 filtration's terminal row gives
     the FP32 terminal-only kernel's moments (the same paths, summed in another order), and both
