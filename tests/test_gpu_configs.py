@@ -86,10 +86,13 @@ def test_c2_fp32_timed_kernel_against_oracle(oracle):
     assert abs(pay_g - pay_r) <= 1e-6 * abs(pay_r), (pay_g, pay_r)
 
 
-def test_c2_model_fp32_pseudo_random_against_oracle(oracle):
-    """The same Heston RK model in FP32 with the reference's default generator, pseudo-random
-    ChaCha8 draws (path s seeded seed + s), whose inverse normal is formed from the draw's two
-    words: 4096 paths x 1024 steps, every value against the oracle's FP64 step applied to the
+@pytest.mark.parametrize("rng", ["pseudo", "shift"])
+def test_c2_model_fp32_words_form_against_oracle(oracle, rng):
+    """The same Heston RK model in FP32 with the reference's own generators: pseudo-random ChaCha8
+    draws (path s seeded seed + s; the reference's default), and Sobol points with the reference's
+    per-path f64 random shift (src/rng/sobol.rs:62-79). Both FP32 kernels form the inverse normal
+    from the uniform held in two words (the draw, or the shifted point as an exact 53-bit
+    fraction). 4096 paths x 1024 steps: every value against the oracle's FP64 step applied to the
     kernel's own row within one FP32 step's rounding (the words form's budget in
     tests/fp32_bounds.py), the first 64 steps against the propagated bound, and the terminal
     payoff mean within 1e-6 of the oracle's."""
@@ -97,15 +100,23 @@ def test_c2_model_fp32_pseudo_random_against_oracle(oracle):
     import fp32_bounds as B
     S, seed = 4096, 42
     times = np.linspace(0.0, 1.0, FULL_N + 1)
-    print("case: c2 model fp32 pseudo S=%d N=%d" % (S, FULL_N), flush=True)
-    plan = sde_sim_rs.Plan(HESTON, times, S, INIT, "pseudo", "runge-kutta", precision="f32", seed=seed)
-    assert "sdesim_rk_f32_pseudo_filt" in plan.source
+    dims = 2 * FULL_N
+    print("case: c2 model fp32 %s S=%d N=%d" % (rng, S, FULL_N), flush=True)
+    gen = "pseudo" if rng == "pseudo" else "sobol"
+    plan = sde_sim_rs.Plan(HESTON, times, S, INIT, gen, "runge-kutta", precision="f32", seed=seed, scramble="shift")
+    assert ("sdesim_rk_f32_%s_filt" % ("pseudo" if rng == "pseudo" else "sobolshift")) in plan.source
     f = plan.new_filtration()
     plan.run(f)
     torch.cuda.synchronize()
     gpu = f.cpu().numpy().astype(np.float64).transpose(2, 0, 1)
-    ref = oracle.simulate(HESTON, times, S, INIT, scheme="runge-kutta", rng="pseudo", seed=seed)
-    y = np.stack([oracle.pseudo_f64(seed + s, 2 * FULL_N) for s in range(S)])
+    ref = oracle.simulate(HESTON, times, S, INIT, scheme="runge-kutta", rng=gen, seed=seed, scramble="shift")
+    draws = np.stack([oracle.pseudo_f64(seed + s, dims) for s in range(S)])
+    if rng == "pseudo":
+        y = draws
+    else:
+        # The reference's composition: frac(point 2^-32 + shift), point index 5 + s.
+        yx = oracle.sobol_points(dims, S, 5).astype(np.float64) * 2.0 ** -32 + draws
+        y = yx - np.trunc(yx)
     X, E = B.heston_rk_bound(y, times, draws="words")
     _close(X, ref, 1e-12, 1e-12)
     n = 65
